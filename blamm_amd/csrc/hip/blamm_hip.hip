@@ -1,0 +1,628 @@
+// libblamm_hip.so: the C ABI declared in include/blamm_hip.h.
+//
+// Replaces the reference's HAVE_CUDA scan worker (scanThreadCUBLAS,
+// pwmscan.cpp:297-437) and kernel.cu.  Host work here is limited to laying the
+// library out for the MFMA kernels (column blocks, fragment swizzle, bf16
+// split, proven filter bands) and to moving buffers; scoring happens on the
+// device in scan_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/blamm_hip.h"
+#include "scan_kernels.hpp"
+
+using namespace blamm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(e_ == hipErrorOutOfMemory ? BLAMM_ERR_NOMEM : BLAMM_ERR_DEVICE,       \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+// Column blocks held in registers per wave.
+constexpr int kNbBf16 = 2;
+constexpr int kNbFp32 = 4;
+
+struct ClassInfo {
+    int nch;          // chunks of 4 positions
+    int blk_begin;    // first device block
+    int nblk;         // blocks (multiple of NB)
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Grow to at least `want` elements (contents are not preserved).
+    hipError_t reserve(size_t want)
+    {
+        if (want <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = std::max<size_t>(want, 1);
+        return e;
+    }
+};
+
+uint16_t bf16_rne(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7FFFu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+float bf16_to_float(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+float round_down(double d)
+{
+    float f = (float)d;
+    if ((double)f > d) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
+float round_up(double d)
+{
+    float f = (float)d;
+    if ((double)f < d) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+// gamma_n = n u / (1 - n u): bound factor for any summation order of n terms.
+double gamma_n(double n, double u) { return n * u / (1.0 - n * u); }
+
+}  // namespace
+
+struct blamm_hip_ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+    // resident library
+    bool have_lib = false;
+    int precision = 0;
+    int ncols = 0;
+    int nslots = 0;
+    std::vector<ClassInfo> classes;
+    double max_filter_band = 0.0, max_score_error = 0.0;
+    DevBuf<uint8_t> frag_a, frag_b;
+    DevBuf<float> thr_lo, slot_thr, slot_seq_lo, slot_seq_hi, slot_w;
+    DevBuf<int> slot_len;
+    DevBuf<uint32_t> slot_col;
+
+    // scratch
+    DevBuf<uint2> cand;
+    DevBuf<unsigned long long> counters;
+
+    // host-buffer staging
+    DevBuf<uint64_t> packed, lcmask, runs, nhits_dev;
+    DevBuf<blamm_hit> hits;
+
+    // last scan
+    bool pending = false;
+    uint64_t pending_hit_cap = 0;
+    hipStream_t pending_stream = nullptr;
+    blamm_scan_stats stats{};
+};
+
+namespace {
+
+template <int NCH, bool MASK>
+hipError_t launch_filter_t(bool bf, const FilterArgs& a, uint32_t grid, hipStream_t s)
+{
+    if (bf)
+        hipLaunchKernelGGL((filter_bf16x2_kernel<NCH, kNbBf16, MASK>), dim3(grid), dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((filter_fp32_kernel<NCH, kNbFp32, MASK>), dim3(grid), dim3(kThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+template <bool MASK>
+hipError_t launch_filter_nch(bool bf, int nch, const FilterArgs& a, uint32_t grid, hipStream_t s)
+{
+    switch (nch) {
+    case 1: return launch_filter_t<1, MASK>(bf, a, grid, s);
+    case 2: return launch_filter_t<2, MASK>(bf, a, grid, s);
+    case 3: return launch_filter_t<3, MASK>(bf, a, grid, s);
+    case 4: return launch_filter_t<4, MASK>(bf, a, grid, s);
+    case 5: return launch_filter_t<5, MASK>(bf, a, grid, s);
+    case 6: return launch_filter_t<6, MASK>(bf, a, grid, s);
+    case 7: return launch_filter_t<7, MASK>(bf, a, grid, s);
+    case 8: return launch_filter_t<8, MASK>(bf, a, grid, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_filter(bool bf, bool mask, int nch, const FilterArgs& a, uint32_t grid, hipStream_t s)
+{
+    return mask ? launch_filter_nch<true>(bf, nch, a, grid, s) : launch_filter_nch<false>(bf, nch, a, grid, s);
+}
+
+__global__ void store_count_kernel(const unsigned long long* counters, uint64_t* out)
+{
+    if (threadIdx.x == 0) *out = counters[kCntHits];
+}
+
+}  // namespace
+
+extern "C" {
+
+int blamm_hip_abi_version(void) { return BLAMM_HIP_ABI_VERSION; }
+
+const char* blamm_hip_last_error(void) { return g_last_error.c_str(); }
+
+uint64_t blamm_hip_packed_words(uint64_t nbases) { return (nbases + 31) / 32 + 8; }
+
+uint64_t blamm_hip_mask_words(uint64_t nbases) { return (nbases + 63) / 64 + 8; }
+
+int blamm_hip_device_count(int* count)
+{
+    if (!count) return fail(BLAMM_ERR_ARG, "device_count: null pointer");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return BLAMM_OK;
+}
+
+int blamm_hip_create(int device, blamm_hip_ctx** out)
+{
+    if (!out) return fail(BLAMM_ERR_ARG, "create: null output pointer");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(BLAMM_ERR_DEVICE, "no HIP device found");
+    if (device < 0 || device >= n) return fail(BLAMM_ERR_ARG, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    std::string arch = prop.gcnArchName;
+    if (arch.rfind("gfx950", 0) != 0)
+        return fail(BLAMM_ERR_DEVICE, "libblamm_hip.so is built for gfx950, device is " + arch);
+    blamm_hip_ctx* c = new blamm_hip_ctx();
+    c->device = device;
+    c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    if (e == hipSuccess) e = c->counters.reserve(kCntSlots);
+    if (e != hipSuccess) {
+        blamm_hip_destroy(c);
+        return fail(BLAMM_ERR_DEVICE, std::string("create: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return BLAMM_OK;
+}
+
+int blamm_hip_destroy(blamm_hip_ctx* c)
+{
+    if (!c) return BLAMM_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->frag_a.release(); c->frag_b.release(); c->thr_lo.release(); c->slot_thr.release();
+    c->slot_seq_lo.release(); c->slot_seq_hi.release(); c->slot_w.release();
+    c->slot_len.release(); c->slot_col.release(); c->cand.release(); c->counters.release();
+    c->packed.release(); c->lcmask.release(); c->runs.release(); c->nhits_dev.release();
+    c->hits.release();
+    for (int i = 0; i < 4; ++i)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BLAMM_OK;
+}
+
+int blamm_hip_set_library(blamm_hip_ctx* c, const float* P, int ldp, int ncols,
+                          const int* lengths, const float* thresholds, int precision)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "set_library: null context");
+    if (!P || !lengths || !thresholds || ncols <= 0)
+        return fail(BLAMM_ERR_ARG, "set_library: empty library");
+    if (precision != BLAMM_PREC_FP32 && precision != BLAMM_PREC_BF16X2)
+        return fail(BLAMM_ERR_ARG, "set_library: unknown precision");
+    int lmax = 0;
+    for (int j = 0; j < ncols; ++j) {
+        if (lengths[j] < 1 || lengths[j] > BLAMM_MAX_MOTIF_LEN)
+            return fail(BLAMM_ERR_ARG, "set_library: motif length of column " + std::to_string(j) +
+                                           " outside 1.." + std::to_string(BLAMM_MAX_MOTIF_LEN));
+        lmax = std::max(lmax, lengths[j]);
+        if (std::isnan(thresholds[j]))
+            return fail(BLAMM_ERR_ARG, "set_library: NaN threshold in column " + std::to_string(j));
+    }
+    if (ldp < 4 * lmax) return fail(BLAMM_ERR_ARG, "set_library: ldp < 4 * max motif length");
+    for (int j = 0; j < ncols; ++j)
+        for (int k = 0; k < 4 * lengths[j]; ++k)
+            if (!std::isfinite(P[(size_t)j * ldp + k]))
+                return fail(BLAMM_ERR_ARG, "set_library: non-finite weight in column " + std::to_string(j) +
+                                               " (PSEUDOCOUNT must keep every PWM entry finite)");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+
+    const bool bf = precision == BLAMM_PREC_BF16X2;
+    const int cols_per_blk = bf ? kBf16Cols : kFp32Cols;
+    const int nb = bf ? kNbBf16 : kNbFp32;
+
+    // Columns by non-decreasing length (stable); consecutive runs of
+    // cols_per_blk form a block whose chunk count is set by its longest column.
+    std::vector<int> order(ncols);
+    for (int j = 0; j < ncols; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return lengths[x] < lengths[y]; });
+    std::vector<std::vector<int>> blocks;   // each: cols_per_blk entries, -1 = empty
+    std::vector<int> blk_nch;
+    for (int s = 0; s < ncols; s += cols_per_blk) {
+        std::vector<int> b(cols_per_blk, -1);
+        int lm = 0;
+        for (int k = 0; k < cols_per_blk && s + k < ncols; ++k) {
+            b[k] = order[s + k];
+            lm = std::max(lm, lengths[order[s + k]]);
+        }
+        blocks.push_back(b);
+        blk_nch.push_back((lm + 3) / 4);
+    }
+    // Classes of equal chunk count, each padded to a multiple of NB blocks.
+    std::vector<std::vector<int>> dev_blocks;
+    std::vector<ClassInfo> classes;
+    for (size_t i = 0; i < blocks.size();) {
+        size_t e = i;
+        while (e < blocks.size() && blk_nch[e] == blk_nch[i]) ++e;
+        ClassInfo ci;
+        ci.nch = blk_nch[i];
+        ci.blk_begin = (int)dev_blocks.size();
+        for (size_t k = i; k < e; ++k) dev_blocks.push_back(blocks[k]);
+        while ((dev_blocks.size() - ci.blk_begin) % nb) dev_blocks.push_back(std::vector<int>(cols_per_blk, -1));
+        ci.nblk = (int)dev_blocks.size() - ci.blk_begin;
+        classes.push_back(ci);
+        i = e;
+    }
+    const int nblk = (int)dev_blocks.size();
+    const int nslots = nblk * cols_per_blk;
+
+    std::vector<float> thr_lo(nslots, INFINITY), s_thr(nslots, INFINITY), s_lo(nslots, INFINITY),
+        s_hi(nslots, INFINITY), s_w((size_t)nslots * kMaxLen * 4, 0.f);
+    std::vector<int> s_len(nslots, 1);
+    std::vector<uint32_t> s_col(nslots, 0);
+    std::vector<uint16_t> hi16((size_t)nslots * kMaxLen * 4, 0), lo16((size_t)nslots * kMaxLen * 4, 0);
+    double max_band = 0.0, max_err = 0.0;
+    const double u24 = std::ldexp(1.0, -24), u53 = std::ldexp(1.0, -53);
+    for (int bi = 0; bi < nblk; ++bi) {
+        int nch = 0;
+        for (const ClassInfo& ci : classes)
+            if (bi >= ci.blk_begin && bi < ci.blk_begin + ci.nblk) nch = ci.nch;
+        for (int k = 0; k < cols_per_blk; ++k) {
+            const int col = dev_blocks[bi][k];
+            if (col < 0) continue;
+            const int slot = bi * cols_per_blk + k;
+            const int L = lengths[col];
+            const float* w = P + (size_t)col * ldp;
+            double sum_max = 0.0, sum_split = 0.0, rep_err = 0.0;
+            for (int p = 0; p < L; ++p) {
+                double mx = 0.0, mxs = 0.0, me = 0.0;
+                for (int b = 0; b < 4; ++b) {
+                    const float x = w[4 * p + b];
+                    s_w[((size_t)slot * kMaxLen + p) * 4 + b] = x;
+                    mx = std::max(mx, (double)std::fabs(x));
+                    if (bf) {
+                        const uint16_t h = bf16_rne(x);
+                        const float hf = bf16_to_float(h);
+                        const uint16_t l = bf16_rne(x - hf);   // x - hf is exact in FP32
+                        const float lf = bf16_to_float(l);
+                        hi16[((size_t)slot * kMaxLen + p) * 4 + b] = h;
+                        lo16[((size_t)slot * kMaxLen + p) * 4 + b] = l;
+                        me = std::max(me, std::fabs((double)x - (double)hf - (double)lf));
+                        mxs = std::max(mxs, std::fabs((double)hf) + std::fabs((double)lf));
+                    } else {
+                        mxs = std::max(mxs, (double)std::fabs(x));
+                    }
+                }
+                sum_max += mx;
+                sum_split += mxs;
+                rep_err += me;
+            }
+            // |FP64 left-to-right sum - exact sum| (binary64, L-1 additions)
+            const double fp64_err = gamma_n(L, u53) * sum_max;
+            // MFMA filter band: representation error of the split plus any
+            // summation order of every product the window's dot product sees
+            // (32 per chunk for hi+lo, 16 per chunk for FP32), with unit
+            // roundoff 2^-22 to allow for truncating accumulation.
+            const double nterms = bf ? 32.0 * nch : 16.0 * nch;
+            const double band = 2.0 * (rep_err + gamma_n(nterms, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
+            // Sequential FP32 band used by finalize (RNE adds, L-1 of them).
+            const double seq = 2.0 * (gamma_n(L, u24) * sum_max + fp64_err) + 1e-30;
+            const double t = (double)thresholds[col];
+            thr_lo[slot] = round_down(t - band);
+            s_thr[slot] = thresholds[col];
+            s_lo[slot] = round_down(t - seq);
+            s_hi[slot] = round_up(t + seq);
+            s_len[slot] = L;
+            s_col[slot] = (uint32_t)col;
+            max_band = std::max(max_band, band);
+            max_err = std::max(max_err, seq);
+        }
+    }
+
+    // Fragment swizzle.
+    std::vector<uint8_t> fa, fb;
+    if (bf) {
+        // [block][chunk][lane] uint4: lane l holds column l%32, positions
+        // 4j + 2(l/32) + {0,1}, 4 bases each, as bf16 pairs.
+        fa.assign((size_t)nblk * kMaxChunks * 64 * 16, 0);
+        fb.assign(fa.size(), 0);
+        for (int bi = 0; bi < nblk; ++bi)
+            for (int j = 0; j < kMaxChunks; ++j)
+                for (int l = 0; l < 64; ++l) {
+                    const int slot = bi * kBf16Cols + (l & 31);
+                    uint16_t vh[8], vl[8];
+                    for (int i = 0; i < 8; ++i) {
+                        const int p = 4 * j + 2 * (l >> 5) + i / 4, b = i % 4;
+                        vh[i] = hi16[((size_t)slot * kMaxLen + p) * 4 + b];
+                        vl[i] = lo16[((size_t)slot * kMaxLen + p) * 4 + b];
+                    }
+                    const size_t off = (((size_t)bi * kMaxChunks + j) * 64 + l) * 16;
+                    std::memcpy(&fa[off], vh, 16);
+                    std::memcpy(&fb[off], vl, 16);
+                }
+    } else {
+        // [block][position][lane] float: lane l holds W_{col l%16}[p][l/16].
+        fa.assign((size_t)nblk * kMaxLen * 64 * 4, 0);
+        for (int bi = 0; bi < nblk; ++bi)
+            for (int p = 0; p < kMaxLen; ++p)
+                for (int l = 0; l < 64; ++l) {
+                    const int slot = bi * kFp32Cols + (l & 15);
+                    const float x = s_w[((size_t)slot * kMaxLen + p) * 4 + (l >> 4)];
+                    std::memcpy(&fa[(((size_t)bi * kMaxLen + p) * 64 + l) * 4], &x, 4);
+                }
+        fb.assign(16, 0);
+    }
+
+    c->have_lib = false;
+    HIP_TRY(c->frag_a.reserve(fa.size()));
+    HIP_TRY(c->frag_b.reserve(fb.size()));
+    HIP_TRY(c->thr_lo.reserve(nslots));
+    HIP_TRY(c->slot_thr.reserve(nslots));
+    HIP_TRY(c->slot_seq_lo.reserve(nslots));
+    HIP_TRY(c->slot_seq_hi.reserve(nslots));
+    HIP_TRY(c->slot_w.reserve(s_w.size()));
+    HIP_TRY(c->slot_len.reserve(nslots));
+    HIP_TRY(c->slot_col.reserve(nslots));
+    HIP_TRY(hipMemcpy(c->frag_a.p, fa.data(), fa.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->frag_b.p, fb.data(), fb.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->thr_lo.p, thr_lo.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_thr.p, s_thr.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_seq_lo.p, s_lo.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_seq_hi.p, s_hi.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_w.p, s_w.data(), s_w.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_len.p, s_len.data(), nslots * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_col.p, s_col.data(), nslots * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->precision = precision;
+    c->ncols = ncols;
+    c->nslots = nslots;
+    c->classes = classes;
+    c->max_filter_band = max_band;
+    c->max_score_error = max_err;
+    c->have_lib = true;
+    return BLAMM_OK;
+}
+
+int blamm_hip_library_bounds(blamm_hip_ctx* c, double* band, double* err)
+{
+    if (!c || !c->have_lib) return fail(BLAMM_ERR_STATE, "library_bounds: no library");
+    if (band) *band = c->max_filter_band;
+    if (err) *err = c->max_score_error;
+    return BLAMM_OK;
+}
+
+int blamm_hip_reserve(blamm_hip_ctx* c, uint64_t n)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "reserve: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(c->cand.reserve(n));
+    return BLAMM_OK;
+}
+
+int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t packed_words,
+                          const uint64_t* d_lcmask, uint64_t mask_words, uint64_t nbases,
+                          const uint64_t* d_run_starts, uint64_t nruns, blamm_hit* d_hits,
+                          uint64_t cap, uint64_t* d_nhits, void* stream)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "scan: null context");
+    if (!c->have_lib) return fail(BLAMM_ERR_STATE, "scan: no library uploaded");
+    if (c->pending) return fail(BLAMM_ERR_STATE, "scan: previous scan not finished");
+    if (nbases >= (1ull << 32) - 512) return fail(BLAMM_ERR_ARG, "scan: nbases must be < 2^32 - 512 per call");
+    if (nbases > 0 && !d_packed) return fail(BLAMM_ERR_ARG, "scan: null packed stream");
+    if (nbases > 0 && packed_words < blamm_hip_packed_words(nbases))
+        return fail(BLAMM_ERR_ARG, "scan: packed buffer shorter than blamm_hip_packed_words(nbases)");
+    if (d_lcmask && mask_words < blamm_hip_mask_words(nbases))
+        return fail(BLAMM_ERR_ARG, "scan: mask buffer shorter than blamm_hip_mask_words(nbases)");
+    if (nruns > 0 && !d_run_starts) return fail(BLAMM_ERR_ARG, "scan: null run starts");
+    if (nruns >= (1ull << 32)) return fail(BLAMM_ERR_ARG, "scan: too many runs");
+    if (cap > 0 && !d_hits) return fail(BLAMM_ERR_ARG, "scan: null hit buffer");
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    if (c->cand.n == 0) HIP_TRY(c->cand.reserve(1u << 20));
+
+    const bool bf = c->precision == BLAMM_PREC_BF16X2;
+    const bool mask = d_lcmask != nullptr;
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, c->counters.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[0], s));
+    if (nbases > 0) {
+        const uint32_t tile = bf ? kBf16Tile : kFp32Tile;
+        const uint32_t ntiles = (uint32_t)((nbases + tile - 1) / tile);
+        const int nb = bf ? kNbBf16 : kNbFp32;
+        for (const ClassInfo& ci : c->classes) {
+            FilterArgs a;
+            a.packed = d_packed;
+            a.lcmask = d_lcmask;
+            a.nbases = (uint32_t)nbases;
+            a.ntiles = ntiles;
+            a.ngroups = (uint32_t)(ci.nblk / nb);
+            const uint64_t target = (uint64_t)c->num_cu * 16;
+            uint64_t ranges = (target + a.ngroups - 1) / a.ngroups;
+            ranges = std::max<uint64_t>(1, std::min<uint64_t>(ranges, ntiles));
+            a.tiles_per_wave = (uint32_t)((ntiles + ranges - 1) / ranges);
+            ranges = (ntiles + a.tiles_per_wave - 1) / a.tiles_per_wave;
+            a.nwaves = (uint32_t)(ranges * a.ngroups);
+            a.blk_begin = (uint32_t)ci.blk_begin;
+            a.frag_a = c->frag_a.p;
+            a.frag_b = c->frag_b.p;
+            a.thr_lo = c->thr_lo.p;
+            a.cand = c->cand.p;
+            a.counters = c->counters.p;
+            a.cand_cap = c->cand.n;
+            const uint32_t grid = (a.nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
+            hipError_t e = launch_filter(bf, mask, ci.nch, a, grid, s);
+            if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("filter launch: ") + hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(hipEventRecord(c->ev[1], s));
+    {
+        FinalizeArgs f;
+        f.cand = c->cand.p;
+        f.counters = c->counters.p;
+        f.counters_rw = c->counters.p;
+        f.cand_cap = c->cand.n;
+        f.packed = d_packed;
+        f.lcmask = d_lcmask;
+        f.nbases = (uint32_t)nbases;
+        f.run_starts = d_run_starts;
+        f.nruns = (uint32_t)nruns;
+        f.slot_len = c->slot_len.p;
+        f.slot_w = c->slot_w.p;
+        f.slot_thr = c->slot_thr.p;
+        f.slot_seq_lo = c->slot_seq_lo.p;
+        f.slot_seq_hi = c->slot_seq_hi.p;
+        f.slot_col = c->slot_col.p;
+        f.hits = d_hits;
+        f.hit_cap = cap;
+        const uint32_t grid = (uint32_t)c->num_cu * 8;
+        hipLaunchKernelGGL(finalize_kernel, dim3(grid), dim3(kThreads), 0, s, f);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(c->ev[2], s));
+    if (d_nhits) {
+        hipLaunchKernelGGL(store_count_kernel, dim3(1), dim3(64), 0, s, c->counters.p, d_nhits);
+        HIP_TRY(hipGetLastError());
+    }
+    c->pending = true;
+    c->pending_hit_cap = cap;
+    c->pending_stream = s;
+    return BLAMM_OK;
+}
+
+int blamm_hip_scan_finish(blamm_hip_ctx* c, uint64_t* nhits)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "scan_finish: null context");
+    if (!c->pending) return fail(BLAMM_ERR_STATE, "scan_finish: no scan in flight");
+    c->pending = false;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->pending_stream));
+    unsigned long long cnt[kCntSlots];
+    HIP_TRY(hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    float ms_f = 0.f, ms_all = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms_f, c->ev[0], c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]));
+    c->stats.candidates = cnt[kCntCand];
+    c->stats.band_rescored = cnt[kCntBand];
+    c->stats.hits = cnt[kCntHits];
+    c->stats.kernel_ms = ms_all;
+    c->stats.filter_ms = ms_f;
+    if (nhits) *nhits = cnt[kCntHits];
+    if (cnt[kCntCand] > c->cand.n) {
+        // Grow the scratch so that a retry holds every candidate.
+        HIP_TRY(c->cand.reserve((size_t)(cnt[kCntCand] + cnt[kCntCand] / 4 + 1024)));
+        if (nhits) *nhits = 0;
+        return fail(BLAMM_ERR_OVERFLOW, "scan: candidate scratch overflowed (grown; retry the scan)");
+    }
+    if (cnt[kCntHits] > c->pending_hit_cap)
+        return fail(BLAMM_ERR_OVERFLOW, "scan: hit capacity too small");
+    return BLAMM_OK;
+}
+
+int blamm_hip_last_stats(blamm_hip_ctx* c, blamm_scan_stats* st)
+{
+    if (!c || !st) return fail(BLAMM_ERR_ARG, "last_stats: null pointer");
+    *st = c->stats;
+    return BLAMM_OK;
+}
+
+int blamm_hip_scan(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
+                   const uint64_t* run_starts, uint64_t nruns, blamm_hit* hits, uint64_t cap,
+                   uint64_t* nhits)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "scan: null context");
+    if (!nhits) return fail(BLAMM_ERR_ARG, "scan: null nhits");
+    if (!c->have_lib) return fail(BLAMM_ERR_STATE, "scan: no library uploaded");
+    if (nbases > 0 && !packed) return fail(BLAMM_ERR_ARG, "scan: null packed stream");
+    if (cap > 0 && !hits) return fail(BLAMM_ERR_ARG, "scan: null hit buffer");
+    if (nbases >= (1ull << 32) - 512) return fail(BLAMM_ERR_ARG, "scan: nbases must be < 2^32 - 512 per call");
+    HIP_TRY(hipSetDevice(c->device));
+    const uint64_t pw = blamm_hip_packed_words(nbases), mw = blamm_hip_mask_words(nbases);
+    HIP_TRY(c->packed.reserve(pw));
+    HIP_TRY(hipMemsetAsync(c->packed.p, 0, pw * 8, c->stream));
+    if (nbases > 0)
+        HIP_TRY(hipMemcpyAsync(c->packed.p, packed, ((nbases + 31) / 32) * 8, hipMemcpyHostToDevice, c->stream));
+    if (lcmask) {
+        HIP_TRY(c->lcmask.reserve(mw));
+        HIP_TRY(hipMemsetAsync(c->lcmask.p, 0, mw * 8, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->lcmask.p, lcmask, ((nbases + 63) / 64) * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_TRY(c->runs.reserve(nruns + 1));
+    if (nruns > 0)
+        HIP_TRY(hipMemcpyAsync(c->runs.p, run_starts, nruns * 8, hipMemcpyHostToDevice, c->stream));
+    uint64_t dcap = std::min<uint64_t>(cap, std::max<uint64_t>(c->hits.n, 1u << 16));
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        HIP_TRY(c->hits.reserve(std::max<uint64_t>(dcap, 1)));
+        int rc = blamm_hip_scan_device(c, c->packed.p, pw, lcmask ? c->lcmask.p : nullptr, mw, nbases,
+                                       c->runs.p, nruns, c->hits.p, dcap, nullptr, c->stream);
+        if (rc != BLAMM_OK) return rc;
+        uint64_t n = 0;
+        rc = blamm_hip_scan_finish(c, &n);
+        if (rc == BLAMM_OK) {
+            if (n > 0) HIP_TRY(hipMemcpy(hits, c->hits.p, n * sizeof(blamm_hit), hipMemcpyDeviceToHost));
+            *nhits = n;
+            return BLAMM_OK;
+        }
+        if (rc != BLAMM_ERR_OVERFLOW) return rc;
+        if (n > dcap) {   // hit buffer too small
+            if (n > cap) {
+                *nhits = n;
+                return fail(BLAMM_ERR_OVERFLOW, "scan: hit capacity too small");
+            }
+            dcap = n;
+        }
+    }
+    return fail(BLAMM_ERR_DEVICE, "scan: capacity retries exhausted");
+}
+
+}  // extern "C"

@@ -1,0 +1,347 @@
+// Device kernels of the MI355X scan engine (gfx950, wave64).
+//
+// The scan computes, for every window q of the packed stream and every PWM
+// column c, score(q,c) = sum_p W_c[p][base(q+p)].  The reference expresses this
+// as R = S * P with S the one-hot strip matrix (SeqMatrix, sequence.cpp:299-340)
+// and P the pattern matrix (motif.cpp:542-564), one sgemm per column tile and
+// per offset (pwmscan.cpp:385-389), followed by a threshold/compaction pass over
+// R (kernel.cu:21-45).  Here the one-hot operand is built in registers straight
+// from 2-bit packed bases, the product runs on MFMA with FP32 accumulation, and
+// the threshold test plus hit compaction run in the epilogue, so R never
+// reaches HBM.  The MFMA score only filters: a window becomes a candidate when
+// it clears thr - band (band = proven bound on |MFMA score - FP64 score|).
+// finalize_kernel then applies the run-boundary rule and decides every
+// candidate exactly against the FP64 left-to-right sum.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace blamm {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWaveSize = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kThreads = kWaveSize * kWavesPerBlock;
+constexpr int kMaxLen = 32;          // positions per column slot
+constexpr int kMaxChunks = kMaxLen / 4;
+constexpr int kBf16Cols = 32;        // columns per bf16 column block (MFMA N)
+constexpr int kBf16Tile = 32;        // windows per bf16 tile (MFMA M)
+constexpr int kFp32Cols = 16;        // columns per fp32 column block
+constexpr int kFp32Tile = 16;        // windows per fp32 tile
+
+// Counter slots in device memory.
+enum { kCntCand = 0, kCntHits = 1, kCntBand = 2, kCntSlots = 4 };
+
+// 64 consecutive bits of a little-endian bit stream starting at bit `bit`.
+__device__ __forceinline__ uint64_t load_bits64(const uint64_t* __restrict__ words, uint64_t bit)
+{
+    const uint64_t wi = bit >> 6;
+    const uint32_t sh = (uint32_t)(bit & 63);
+    const uint64_t w0 = words[wi];
+    const uint64_t w1 = words[wi + 1];
+    return sh ? ((w0 >> sh) | (w1 << (64 - sh))) : w0;
+}
+
+// Reserve `cnt` consecutive slots of a device counter for the set lanes of
+// `mask` and return this lane's slot (only meaningful for set lanes).  One
+// atomic per wave; the counter is 64-bit so it never wraps.
+__device__ __forceinline__ uint64_t wave_reserve(unsigned long long* counter, uint64_t mask)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t cnt = (uint32_t)__popcll(mask);
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(counter, (unsigned long long)cnt);
+    base = __shfl(base, 0);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    return (uint64_t)base + rank;
+}
+
+struct FilterArgs {
+    const uint64_t* packed;      // 2-bit codes
+    const uint64_t* lcmask;      // lowercase bits (MASK variants only)
+    uint32_t nbases;
+    uint32_t ntiles;             // window tiles
+    uint32_t tiles_per_wave;
+    uint32_t ngroups;            // column groups (NB blocks each) of this class
+    uint32_t nwaves;             // ngroups * window ranges
+    uint32_t blk_begin;          // first column block of this class
+    const void* frag_a;          // bf16: hi fragments; fp32: weights
+    const void* frag_b;          // bf16: lo fragments
+    const float* thr_lo;         // per slot: filter threshold (thr - band, rounded down)
+    uint2* cand;                 // (window, slot)
+    unsigned long long* counters;
+    uint64_t cand_cap;
+};
+
+// Emit (window, slot) for every accumulator entry clearing its threshold.
+// Called only when the wave holds at least one such entry.
+template <int NREG>
+__device__ __forceinline__ void emit_candidates(const FilterArgs& a, const float* acc, float thr,
+                                                uint32_t q_of_reg0, const uint32_t* row_of_reg,
+                                                uint32_t slot)
+{
+#pragma unroll
+    for (int r = 0; r < NREG; ++r) {
+        const uint32_t q = q_of_reg0 + row_of_reg[r];
+        const bool pred = (acc[r] >= thr) && (q < a.nbases);
+        const uint64_t m = __ballot(pred);
+        if (m) {
+            const uint64_t at = wave_reserve(&a.counters[kCntCand], m);
+            if (pred && at < a.cand_cap) a.cand[at] = make_uint2(q, slot);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 filter: W = hi + lo (bf16 each); one-hot operand is exact in bf16.
+// MFMA v_mfma_f32_32x32x16_bf16: A = one-hot (32 windows x 16 = 4 positions x
+// 4 bases), B = PWM chunk (16 x 32 columns), D = 32x32 FP32 scores.
+// Lane l: A rows/ B cols index l%32, K half h = l/32 holds positions 2h, 2h+1
+// of the chunk; D lane l holds column l%32, rows 8(r/4) + 4h + r%4.
+// Each wave keeps NB column blocks (NCH chunks, hi and lo) in registers and
+// streams window tiles through them.
+// ---------------------------------------------------------------------------
+template <int NCH, int NB, bool MASK>
+__global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (wave >= a.nwaves) return;
+    const uint32_t g = wave % a.ngroups;
+    const uint32_t rng = wave / a.ngroups;
+    const uint32_t t0 = rng * a.tiles_per_wave;
+    uint32_t t1 = t0 + a.tiles_per_wave;
+    if (t1 > a.ntiles) t1 = a.ntiles;
+    const uint32_t blk0 = a.blk_begin + g * NB;
+
+    const uint4* __restrict__ fh = (const uint4*)a.frag_a;
+    const uint4* __restrict__ fl = (const uint4*)a.frag_b;
+    bf16x8 bh[NB][NCH], bl[NB][NCH];
+    float th[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+#pragma unroll
+        for (int j = 0; j < NCH; ++j) {
+            const size_t idx = ((size_t)(blk0 + b) * kMaxChunks + j) * 64 + lane;
+            const uint4 h = fh[idx], l = fl[idx];
+            bh[b][j] = __builtin_bit_cast(bf16x8, h);
+            bl[b][j] = __builtin_bit_cast(bf16x8, l);
+        }
+        th[b] = a.thr_lo[(blk0 + b) * kBf16Cols + (lane & 31)];
+    }
+    const uint32_t m = lane & 31, hh = lane >> 5;
+    uint32_t rows[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) rows[r] = 8 * (r / 4) + 4 * hh + (r % 4);
+
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t q0 = t * kBf16Tile;
+        const uint64_t start = (uint64_t)q0 + m + 2 * hh;
+        const uint64_t bits = load_bits64(a.packed, 2 * start);
+        uint64_t mb = 0;
+        if (MASK) mb = load_bits64(a.lcmask, start);
+        f32x16 acc[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
+#pragma unroll
+        for (int j = 0; j < NCH; ++j) {
+            const uint32_t b0 = (uint32_t)(bits >> (8 * j)) & 3u;
+            const uint32_t b1 = (uint32_t)(bits >> (8 * j + 2)) & 3u;
+            uint64_t o0 = 0x3F80ull << (b0 * 16);   // bf16 1.0 in lane b0 of 4
+            uint64_t o1 = 0x3F80ull << (b1 * 16);
+            if (MASK) {
+                if ((mb >> (4 * j)) & 1) o0 = 0;
+                if ((mb >> (4 * j + 1)) & 1) o1 = 0;
+            }
+            const u32x4 av = {(uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32)};
+            const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bh[b][j], acc[b], 0, 0, 0);
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bl[b][j], acc[b], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            float mx = acc[b][0];
+#pragma unroll
+            for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[b][r]);
+            if (__ballot(mx >= th[b])) {
+                float v[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[r] = acc[b][r];
+                emit_candidates<16>(a, v, th[b], q0, rows, (blk0 + b) * kBf16Cols + (lane & 31));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 filter: v_mfma_f32_16x16x4_f32, one motif position per MFMA.
+// A = one-hot (16 windows x 4 bases), B = W_c[p][.] (4 x 16 columns).
+// Lane l: A[l%16][l/16], B[l/16][l%16]; D lane l holds column l%16,
+// rows 4(l/16) + r.
+// ---------------------------------------------------------------------------
+template <int NCH, int NB, bool MASK>
+__global__ __launch_bounds__(kThreads) void filter_fp32_kernel(FilterArgs a)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (wave >= a.nwaves) return;
+    const uint32_t g = wave % a.ngroups;
+    const uint32_t rng = wave / a.ngroups;
+    const uint32_t t0 = rng * a.tiles_per_wave;
+    uint32_t t1 = t0 + a.tiles_per_wave;
+    if (t1 > a.ntiles) t1 = a.ntiles;
+    const uint32_t blk0 = a.blk_begin + g * NB;
+
+    const float* __restrict__ fw = (const float*)a.frag_a;
+    constexpr int NP = 4 * NCH;
+    float bw[NB][NP];
+    float th[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) bw[b][p] = fw[((size_t)(blk0 + b) * kMaxLen + p) * 64 + lane];
+        th[b] = a.thr_lo[(blk0 + b) * kFp32Cols + (lane & 15)];
+    }
+    const uint32_t i = lane & 15, k = lane >> 4;
+    uint32_t rows[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) rows[r] = 4 * k + r;
+
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t q0 = t * kFp32Tile;
+        const uint64_t start = (uint64_t)q0 + i;
+        const uint64_t bits = load_bits64(a.packed, 2 * start);
+        uint64_t mb = 0;
+        if (MASK) mb = load_bits64(a.lcmask, start);
+        f32x4 acc[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const uint32_t base = (uint32_t)(bits >> (2 * p)) & 3u;
+            bool on = (base == k);
+            if (MASK) on = on && !((mb >> p) & 1);
+            const float av = on ? 1.0f : 0.0f;
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bw[b][p], acc[b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const float mx = fmaxf(fmaxf(acc[b][0], acc[b][1]), fmaxf(acc[b][2], acc[b][3]));
+            if (__ballot(mx >= th[b])) {
+                float v[4] = {acc[b][0], acc[b][1], acc[b][2], acc[b][3]};
+                emit_candidates<4>(a, v, th[b], q0, rows, (blk0 + b) * kFp32Cols + (lane & 15));
+            }
+        }
+    }
+}
+
+struct FinalizeArgs {
+    const uint2* cand;
+    const unsigned long long* counters;   // kCntCand read, kCntHits/kCntBand written
+    unsigned long long* counters_rw;
+    uint64_t cand_cap;
+    const uint64_t* packed;
+    const uint64_t* lcmask;               // may be null
+    uint32_t nbases;
+    const uint64_t* run_starts;
+    uint32_t nruns;
+    const int* slot_len;
+    const float* slot_w;                  // [slot][kMaxLen][4] FP32 weights
+    const float* slot_thr;                // the threshold itself
+    const float* slot_seq_lo;             // thr - sequential-FP32 band (rounded down)
+    const float* slot_seq_hi;             // thr + sequential-FP32 band (rounded up)
+    const uint32_t* slot_col;             // caller's column index
+    void* hits;                           // blamm_hit[]
+    uint64_t hit_cap;
+};
+
+struct DevHit {
+    uint64_t pos;
+    uint32_t col;
+    float score;
+};
+
+// Exact decision for each candidate: run-boundary rule (pwmscan.cpp:122-126),
+// then the left-to-right FP32 sum; windows whose FP32 sum lies within its
+// proven band of the threshold are rescored in FP64 (the only FP64 step).
+__global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
+{
+    const uint64_t ncand_raw = a.counters[kCntCand];
+    const uint64_t ncand = ncand_raw < a.cand_cap ? ncand_raw : a.cand_cap;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = (uint64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
+    DevHit* hits = (DevHit*)a.hits;
+    for (uint64_t base = wave * 64; base < ncand; base += nw * 64) {
+        const uint64_t idx = base + lane;
+        bool hit = false;
+        float score = 0.f;
+        uint32_t q = 0, slot = 0;
+        if (idx < ncand) {
+            const uint2 cd = a.cand[idx];
+            q = cd.x;
+            slot = cd.y;
+            const int L = a.slot_len[slot];
+            uint32_t lo = 0, hi = a.nruns;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.run_starts[mid] <= q) lo = mid + 1; else hi = mid;
+            }
+            const uint64_t nxt = lo < a.nruns ? a.run_starts[lo] : (uint64_t)a.nbases;
+            if ((uint64_t)q + (uint64_t)L <= nxt) {
+                const uint64_t bits = load_bits64(a.packed, 2ull * q);
+                const uint64_t mb = a.lcmask ? load_bits64(a.lcmask, q) : 0ull;
+                const float* w = a.slot_w + (size_t)slot * kMaxLen * 4;
+                float s = 0.f;
+                for (int p = 0; p < L; ++p) {
+                    if ((mb >> p) & 1) continue;
+                    s += w[4 * p + ((bits >> (2 * p)) & 3)];
+                }
+                if (s >= a.slot_seq_hi[slot]) {
+                    hit = true;
+                    score = s;
+                } else if (s >= a.slot_seq_lo[slot]) {
+                    double d = 0.0;
+                    for (int p = 0; p < L; ++p) {
+                        if ((mb >> p) & 1) continue;
+                        d += (double)w[4 * p + ((bits >> (2 * p)) & 3)];
+                    }
+                    atomicAdd(&a.counters_rw[kCntBand], 1ull);
+                    if (d >= (double)a.slot_thr[slot]) {
+                        hit = true;
+                        score = (float)d;
+                    }
+                }
+            }
+        }
+        const uint64_t m = __ballot(hit);
+        if (m) {
+            const uint64_t at = wave_reserve(&a.counters_rw[kCntHits], m);
+            if (hit && at < a.hit_cap) {
+                DevHit h;
+                h.pos = q;
+                h.col = a.slot_col[slot];
+                h.score = score;
+                hits[at] = h;
+            }
+        }
+    }
+}
+
+__global__ void zero_counters_kernel(unsigned long long* c)
+{
+    if (threadIdx.x < kCntSlots) c[threadIdx.x] = 0ull;
+}
+
+}  // namespace blamm

@@ -1,0 +1,455 @@
+// The `blamm dict`, `blamm hist` and `blamm scan` commands.
+//
+// Command lines, messages, side files and output formats follow the reference
+// modules (dict.cpp:53-115, hist.cpp:177-266, pwmscan.cpp:456-648).  The scan
+// body differs by design: each manifest group is packed once to 2 bits per
+// base and handed to the MI355X engine (include/blamm_hip.h), which returns the
+// compacted occurrence list; this file only formats it.
+#include <algorithm>
+#include <charconv>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <set>
+#include <sstream>
+#include <thread>
+
+#include "../../../include/blamm_hip.h"
+#include "host.hpp"
+
+namespace blamm {
+
+std::vector<Group> build_dict(const std::string& manifest)
+{
+    std::ifstream in(manifest);
+    if (!in) throw std::runtime_error("Could not open file: " + manifest);
+    std::vector<Group> groups;
+    std::map<std::string, size_t> index;
+    std::vector<std::set<std::string>> files;
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty()) continue;
+        std::istringstream ls(line);
+        std::string name, fasta;
+        ls >> name >> fasta;
+        if (name.empty() || fasta.empty())
+            throw std::runtime_error("File " + manifest + " has incorrect format.\n"
+                                     "Refer to the documention for more information.");
+        if (!std::ifstream(fasta)) throw std::runtime_error("Could not open fasta file: " + fasta);
+        auto it = index.find(name);
+        if (it == index.end()) {
+            it = index.emplace(name, groups.size()).first;
+            groups.emplace_back();
+            groups.back().name = name;
+            files.emplace_back();
+        }
+        files[it->second].insert(fasta);
+    }
+    for (size_t g = 0; g < groups.size(); ++g) {
+        std::cout << "Computing nucleotide composition for " << groups[g].name << " ...";
+        std::cout.flush();
+        groups[g].files.assign(files[g].begin(), files[g].end());
+        const PackedStream s = pack_fasta(groups[g].files, UINT64_MAX);
+        std::array<uint64_t, 4> cnt{{0, 0, 0, 0}};
+        for (uint64_t w = 0; w < s.codes.size(); ++w) {
+            const uint64_t word = s.codes[w];
+            const uint64_t nb = std::min<uint64_t>(32, s.n - 32 * w);
+            for (uint64_t i = 0; i < nb; ++i) ++cnt[(word >> (2 * i)) & 3];
+        }
+        groups[g].counts = cnt;
+        groups[g].totSeqLen = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+        groups[g].seqNames = s.names;
+        std::cout << "\n";
+    }
+    return groups;
+}
+
+namespace {
+
+void dict_usage()
+{
+    std::cout << "Usage: blamm dict [options] sequences.input\n"
+                 "Goal: generate a dictionary for the input sequences\n\n"
+                 " [options]\n  -h\t--help\t\tdisplay help message\n\n";
+}
+
+void hist_usage()
+{
+    std::cout << "Usage: blamm hist [options] motifs.input sequences.input\n"
+                 "Goal: compute PWM score histograms\n\n"
+                 " [options]\n  -h\t--help\t\tdisplay help message\n\n"
+                 " [options arg]\n"
+                 "  -b\t--numbins\tnumber of bins per histogram [default = 250]\n"
+                 "  -t\t--numthreads\tset the number of parallel threads [default = #cores]\n"
+                 "  -l\t--length\tlength of sequence data to analyze [default = 10000000]\n\n"
+                 " [file_options]\n"
+                 "  -H\t--histdir\toutput directory for the histogram file(s) [default = .]\n\n";
+}
+
+void scan_usage()
+{
+    std::cout << "Usage: blamm scan [options] motifs.input sequences.input\n"
+                 "Goal: find PWM matches in sequences\n\n"
+                 " [options]\n"
+                 "  -h\t--help\t\tdisplay help message\n"
+                 "  -c\t--cuda\t\taccepted for compatibility; the MI355X engine is always used\n"
+                 "  -rc\t--revcompl\talso search the reverse strand for occurrences\n\n"
+                 " [options arg]\n"
+                 "  -at\t--absthreshold\tset the minimal absolute score for a motif occurrence\n"
+                 "  -rt\t--relthreshold\tset the minimal relative score [0..1] for a motif occurrence (default = 0.95)\n"
+                 "  -pt\t--pthreshold\tcompute the motif score threshold from p-value [0..1] (default = 1E-4)\n"
+                 "  -t\t--numthreads\tset the number of host threads [default = #cores]\n"
+                 "  -p\t--precision\tbf16x2 (default) or fp32\n"
+                 "  -d\t--device\tHIP device index [default = 0]\n\n"
+                 " [file_options]\n"
+                 "  -H\t--histdir\tdirectory where the histogram file(s) are stored [default = .]\n"
+                 "  -o\t--output\tfilename for the motif occurrences [default = occurences.txt]\n\n";
+}
+
+std::string dir_with_slash(std::string d)
+{
+    if (d.empty() || d.back() != '/') d.push_back('/');
+    return d;
+}
+
+// Append an unsigned integer.
+inline void put_u64(std::string& s, uint64_t v)
+{
+    char b[24];
+    auto r = std::to_chars(b, b + sizeof(b), v);
+    s.append(b, r.ptr);
+}
+
+// Append a float as ostream's default formatting prints it (%g, 6 digits).
+inline void put_score(std::string& s, float f)
+{
+    char b[32];
+    auto r = std::to_chars(b, b + sizeof(b), (double)f, std::chars_format::general, 6);
+    s.append(b, r.ptr);
+}
+
+struct ColumnInfo {
+    std::string name;
+    uint64_t len;
+    char strand;
+};
+
+// One output line per hit: seqName  blamm  motif  pos  pos+L  score  strand  .  .
+// (pwmscan.cpp:88-95); formatted by `threads` workers, written in hit order.
+uint64_t write_hits(std::ostream& os, const std::vector<blamm_hit>& hits, const PackedStream& s,
+                    uint64_t base, const std::vector<ColumnInfo>& cols, const Group& g, int threads)
+{
+    const size_t n = hits.size();
+    if (n == 0) return 0;
+    threads = (int)std::max<size_t>(1, std::min<size_t>((size_t)threads, n / 4096 + 1));
+    std::vector<std::string> out((size_t)threads);
+    std::vector<std::string> err((size_t)threads);
+    auto work = [&](int t) {
+        const size_t b = n * (size_t)t / (size_t)threads, e = n * (size_t)(t + 1) / (size_t)threads;
+        std::string& o = out[(size_t)t];
+        o.reserve((e - b) * 64);
+        for (size_t i = b; i < e; ++i) {
+            const blamm_hit& h = hits[i];
+            uint64_t seq, pos;
+            s.locate(base + h.pos, seq, pos);
+            if (seq >= g.seqNames.size()) {
+                err[(size_t)t] = "sequence index beyond the dictionary of group " + g.name +
+                                 " (is " + g.name + "'s .dict stale?)";
+                return;
+            }
+            const ColumnInfo& c = cols[h.col];
+            o += g.seqNames[seq];
+            o += "\tblamm\t";
+            o += c.name;
+            o += '\t';
+            put_u64(o, pos);
+            o += '\t';
+            put_u64(o, pos + c.len);
+            o += '\t';
+            put_score(o, h.score);
+            o += '\t';
+            o += c.strand;
+            o += "\t.\t.\n";
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    for (const std::string& e : err)
+        if (!e.empty()) throw std::runtime_error(e);
+    for (const std::string& o : out) os.write(o.data(), (std::streamsize)o.size());
+    return n;
+}
+
+struct HipContext {
+    blamm_hip_ctx* ctx = nullptr;
+    ~HipContext()
+    {
+        if (ctx) blamm_hip_destroy(ctx);
+    }
+};
+
+void hip_check(int rc, const char* what)
+{
+    if (rc != BLAMM_OK) throw std::runtime_error(std::string(what) + ": " + blamm_hip_last_error());
+}
+
+}  // namespace
+
+int dict_main(int argc, char** argv)
+{
+    if (argc < 3) {
+        dict_usage();
+        return EXIT_FAILURE;
+    }
+    for (int i = 2; i < argc - 1; ++i) {
+        const std::string a(argv[i]);
+        dict_usage();
+        return (a == "-h" || a == "--help") ? EXIT_SUCCESS : EXIT_FAILURE;
+    }
+    std::cout << "Welcome to blamm -- dictionary model" << std::endl;
+    const std::string manifest(argv[argc - 1]);
+    const std::vector<Group> groups = build_dict(manifest);
+    write_dict(manifest + ".dict", groups);
+    std::cout << "Done!  Dictionary written to " << manifest + ".dict" << std::endl;
+    return EXIT_SUCCESS;
+}
+
+int hist_main(int argc, char** argv)
+{
+    if (argc < 4) {
+        hist_usage();
+        return EXIT_FAILURE;
+    }
+    size_t maxLength = 10000000, numBins = 250;
+    std::string histdir;
+    for (int i = 2; i < argc - 2; ++i) {
+        const std::string a(argv[i]);
+        const bool more = i + 1 < argc - 2;
+        if (a == "-h" || a == "--help") {
+            hist_usage();
+            return EXIT_SUCCESS;
+        } else if ((a == "-l" || a == "--length") && more) {
+            maxLength = (size_t)std::atoll(argv[++i]);
+        } else if ((a == "-b" || a == "--numbins") && more) {
+            numBins = (size_t)std::atoll(argv[++i]);
+            if (numBins < 2) numBins = 2;
+        } else if ((a == "-t" || a == "--numthreads") && more) {
+            ++i;
+        } else if ((a == "-H" || a == "--histdir") && more) {
+            histdir = dir_with_slash(argv[++i]);
+        } else if (a == "-e" || a == "--empirical") {
+            throw std::runtime_error("empirical histograms (-e) are not available in this build");
+        } else {
+            hist_usage();
+            return EXIT_FAILURE;
+        }
+    }
+    std::cout << "Welcome to blamm -- histogram module" << std::endl;
+    const Settings st = Settings::load();
+    const std::vector<Group> groups = read_dict(std::string(argv[argc - 1]) + ".dict");
+    const std::vector<Motif> motifs = load_motifs(argv[argc - 2], false);
+    size_t lmax = 0;
+    for (const Motif& m : motifs) lmax = std::max(lmax, m.size());
+    std::cout << "Loaded " << motifs.size() << " motifs from disk";
+    std::cout << "\nMaximum motif size: " << lmax << std::endl;
+    for (const Group& g : groups) {
+        const std::array<float, 4> bg = g.probabilities(st.pseudocount);
+        std::cout << "Generating histograms for species: " << g.name << "\n";
+        for (const Motif& m : motifs) {
+            std::vector<float> w(4 * m.size());
+            pfm_to_pwm(m, g.counts, st.pseudocount, w.data());
+            const Histogram h = theoretical_histogram(w.data(), m.size(), bg, numBins, maxLength);
+            h.write(histdir, "hist_" + g.name + "_" + m.name, m.name + " (" + g.name + ")");
+        }
+    }
+    return EXIT_SUCCESS;
+}
+
+int scan_main(int argc, char** argv)
+{
+    if (argc < 4) {
+        scan_usage();
+        return EXIT_FAILURE;
+    }
+    bool revCompl = false, absSpec = false, relSpec = false, pSpec = false;
+    float absTh = 0.0f, relTh = 0.95f, pvalue = 0.0001f;
+    size_t numThreads = std::max(1u, std::thread::hardware_concurrency());
+    std::string histdir, outName = "occurrences.txt";
+    int precision = BLAMM_PREC_BF16X2, device = 0;
+    for (int i = 2; i < argc - 2; ++i) {
+        const std::string a(argv[i]);
+        const bool more = i + 1 < argc - 2;
+        if (a == "-h" || a == "--help") {
+            scan_usage();
+            return EXIT_SUCCESS;
+        } else if (a == "-rc" || a == "--revcompl") {
+            revCompl = true;
+        } else if (a == "-s" || a == "--simple") {
+            throw std::runtime_error("the simple scan (-s) is not available in this build");
+        } else if (a == "-c" || a == "--cuda") {
+        } else if ((a == "-at" || a == "--absthreshold") && more) {
+            absSpec = true;
+            absTh = (float)std::atof(argv[++i]);
+        } else if ((a == "-rt" || a == "--relthreshold") && more) {
+            relSpec = true;
+            relTh = (float)std::atof(argv[++i]);
+            if (relTh < 0.0f || relTh > 1.0f)
+                throw std::runtime_error("The relative threshold should be in range [0..1].");
+        } else if ((a == "-pt" || a == "--pthreshold") && more) {
+            pSpec = true;
+            pvalue = (float)std::atof(argv[++i]);
+            if (pvalue < 0.0f || pvalue > 1.0f) throw std::runtime_error("The p-value should be in range [0..1].");
+        } else if ((a == "-t" || a == "--numthreads") && more) {
+            const int t = std::atoi(argv[++i]);
+            if (t < 1) throw std::runtime_error("Number of threads must be a non-zero positive number");
+            numThreads = (size_t)t;
+        } else if ((a == "-H" || a == "--histdir") && more) {
+            histdir = dir_with_slash(argv[++i]);
+        } else if ((a == "-o" || a == "--output") && more) {
+            outName = argv[++i];
+        } else if ((a == "-p" || a == "--precision") && more) {
+            const std::string p(argv[++i]);
+            if (p == "fp32") precision = BLAMM_PREC_FP32;
+            else if (p == "bf16x2") precision = BLAMM_PREC_BF16X2;
+            else throw std::runtime_error("precision must be fp32 or bf16x2");
+        } else if ((a == "-d" || a == "--device") && more) {
+            device = std::atoi(argv[++i]);
+        } else {
+            scan_usage();
+            return EXIT_FAILURE;
+        }
+    }
+    if (!(absSpec || relSpec || pSpec)) relSpec = true;
+    if (absSpec && relSpec) throw std::runtime_error("Specify either the absolute or relative threshold, not both.");
+    if (absSpec && pSpec) throw std::runtime_error("Specify either the absolute or p-value threshold, not both.");
+    if (relSpec && pSpec) throw std::runtime_error("Specify either the relative or p-value threshold, not both.");
+
+    std::cout << "Welcome to blamm -- PWM scan module" << std::endl;
+    const Settings st = Settings::load();
+    st.print();
+    const std::string manifest(argv[argc - 1]);
+    const std::vector<Group> groups = read_dict(manifest + ".dict");
+    std::vector<Motif> motifs = load_motifs(argv[argc - 2], true);
+    size_t lmax = 0;
+    for (const Motif& m : motifs) lmax = std::max(lmax, m.size());
+    std::cout << "Loaded " << motifs.size() << " motifs from disk\n";
+    std::cout << "Maximum motif size: " << lmax << std::endl;
+    if (revCompl) {
+        motifs = add_reverse_complements(motifs);
+        std::cout << "Scanning both forward and reverse strand of the input sequence(s)" << std::endl;
+    } else {
+        std::cout << "Scanning only the forward strand of the input sequence(s)" << std::endl;
+    }
+    for (const Motif& m : motifs)
+        if (m.size() < 1 || m.size() > BLAMM_MAX_MOTIF_LEN)
+            throw std::runtime_error("motif " + m.name + " has length " + std::to_string(m.size()) +
+                                     "; this build scores lengths 1.." + std::to_string(BLAMM_MAX_MOTIF_LEN));
+    if (absSpec) std::cout << "Absolute motif score threshold set to: " << absTh << std::endl;
+    else if (relSpec) std::cout << "Relative motif score threshold set to: " << relTh << std::endl;
+    else std::cout << "P-value motif score threshold set to: " << pvalue << std::endl;
+
+    int ndev = 0;
+    blamm_hip_device_count(&ndev);
+    if (ndev == 0) throw std::runtime_error("HIP error: no devices found. Aborting...");
+    std::cout << "Using 1 GPU device (" << (precision == BLAMM_PREC_FP32 ? "fp32" : "split-bf16")
+              << " MFMA) and " << numThreads << " host thread(s)" << std::endl;
+    HipContext hc;
+    hip_check(blamm_hip_create(device, &hc.ctx), "device setup");
+
+    const size_t ncols = motifs.size();
+    std::vector<ColumnInfo> cols(ncols);
+    std::vector<int> lengths(ncols);
+    for (size_t c = 0; c < ncols; ++c) {
+        cols[c] = {motifs[c].name, motifs[c].size(), motifs[c].rc ? '-' : '+'};
+        lengths[c] = (int)motifs[c].size();
+    }
+    const size_t ldp = 4 * std::max<size_t>(lmax, 1);
+
+    std::ofstream cutoffs("PWMthresholds.txt");
+    std::ofstream os(outName);
+    if (!os) throw std::runtime_error("Cannot write to file: " + outName);
+    uint64_t total = 0;
+    for (const Group& g : groups) {
+        const std::array<float, 4> bgp = g.probabilities(st.pseudocount);
+        {
+            const auto prec = std::cout.precision(3);
+            std::cout << "Scanning species: " << g.name << " [A: " << 100.0f * bgp[0] << "%, C: " << 100.0f * bgp[1]
+                      << "%, G: " << 100.0f * bgp[2] << "%, T: " << 100.0f * bgp[3] << "%]\n";
+            std::cout.precision(prec);
+        }
+        std::vector<float> P(ldp * ncols, 0.0f), thr(ncols);
+        for (size_t c = 0; c < ncols; ++c) pfm_to_pwm(motifs[c], g.counts, st.pseudocount, &P[c * ldp]);
+        std::map<std::string, float> pcut;   // one histogram per base name (fwd and rc share it)
+        for (size_t c = 0; c < ncols; ++c) {
+            const float* w = &P[c * ldp];
+            if (absSpec) {
+                thr[c] = absTh;
+            } else if (relSpec) {
+                const float mx = max_score(w, motifs[c].size()), mn = min_score(w, motifs[c].size());
+                thr[c] = relTh * (mx - mn) + mn;
+            } else {
+                const std::string bn = motifs[c].baseName();
+                auto it = pcut.find(bn);
+                if (it == pcut.end())
+                    it = pcut.emplace(bn, Histogram::load(histdir + "hist_" + g.name + "_" + bn + ".dat").cutoff(pvalue)).first;
+                thr[c] = it->second;
+            }
+        }
+        for (size_t c = 0; c < ncols; ++c) {
+            if (motifs[c].size() > 15) continue;
+            const float* w = &P[c * ldp];
+            cutoffs << g.name << "\t" << motifs[c].name << "\t" << min_score(w, motifs[c].size()) << "\t" << thr[c]
+                    << "\t" << max_score(w, motifs[c].size()) << std::endl;
+        }
+        const PackedStream s = pack_fasta(g.files, g.totSeqLen);
+        hip_check(blamm_hip_set_library(hc.ctx, P.data(), (int)ldp, (int)ncols, lengths.data(), thr.data(), precision),
+                  "library upload");
+        // Segments of the stream (multiples of 64 bases) with a halo of
+        // lmax - 1 bases; hits starting in a halo belong to the next segment.
+        const uint64_t seg = 1ull << 30;
+        const uint64_t halo = lmax - 1;
+        std::vector<blamm_hit> hits(1u << 20);
+        for (uint64_t s0 = 0; s0 < s.n || (s0 == 0 && s.n == 0); s0 += seg) {
+            if (s.n == 0) break;
+            const uint64_t payload = std::min(seg, s.n - s0);
+            const uint64_t len = std::min(s.n - s0, payload + halo);
+            std::vector<uint64_t> runs;
+            for (size_t r = 0; r < s.runStart.size(); ++r)
+                if (s.runStart[r] > s0 && s.runStart[r] < s0 + len) runs.push_back(s.runStart[r] - s0);
+            std::vector<uint64_t> codes(blamm_hip_packed_words(len), 0), lower;
+            std::memcpy(codes.data(), s.codes.data() + s0 / 32, ((len + 31) / 32) * 8);
+            if (s.hasLower()) {
+                lower.assign(blamm_hip_mask_words(len), 0);
+                std::memcpy(lower.data(), s.lower.data() + s0 / 64, ((len + 63) / 64) * 8);
+            }
+            uint64_t nh = 0;
+            int rc;
+            for (;;) {
+                rc = blamm_hip_scan(hc.ctx, codes.data(), s.hasLower() ? lower.data() : nullptr, len, runs.data(),
+                                    runs.size(), hits.data(), hits.size(), &nh);
+                if (rc == BLAMM_ERR_OVERFLOW && nh > hits.size()) {
+                    hits.resize(nh + nh / 8);
+                    continue;
+                }
+                break;
+            }
+            hip_check(rc, "scan");
+            std::vector<blamm_hit> kept;
+            kept.reserve(nh);
+            for (uint64_t i = 0; i < nh; ++i)
+                if (hits[i].pos < payload) kept.push_back(hits[i]);
+            total += write_hits(os, kept, s, s0, cols, g, (int)numThreads);
+        }
+    }
+    os.close();
+    cutoffs.close();
+    std::cout << "\nWrote " << total << " matches to " << outName << ".\n";
+    return EXIT_SUCCESS;
+}
+
+}  // namespace blamm

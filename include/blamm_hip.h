@@ -1,0 +1,156 @@
+/*
+ * blamm_hip.h -- C ABI of libblamm_hip.so, the MI355X scan engine behind
+ * `blamm scan`.
+ *
+ * It replaces the reference's CUDA branch (compiled under HAVE_CUDA,
+ * CMakeLists.txt:57-72, selected by -c/--cuda in pwmscan.cpp:479-480):
+ *   - the per-thread device setup and uploads of scanThreadCUBLAS
+ *     (pwmscan.cpp:297-357: cudaSetDevice, cudaMalloc of P/S/R/thresholds,
+ *     cublasSetVector(P), cublasSetVector(threshold)),
+ *   - the offset loop of cublasSgemm calls (matrix.h:314-323, pwmscan.cpp:385-389),
+ *   - the threshold/compaction kernel kernel_wrapper/filterScore
+ *     (kernel.cu:21-45, declared at pwmscan.cpp:41),
+ *   - the blocking hit read-back (pwmscan.cpp:393-406).
+ *
+ * Plain pointers and sizes only.  Nothing throws across this boundary: every
+ * entry point returns an int status (BLAMM_OK = 0, negative on error) and
+ * blamm_hip_last_error() gives the calling thread's last message.
+ * One context per host thread (thread-compatible, not thread-safe).
+ *
+ * Sequence layout ("packed stream"): the filtered stream of a manifest group
+ * (only ACGTacgt bytes, runs concatenated, sequence.cpp:181-250) as 2-bit codes
+ * A=0 C=1 G=2 T=3, base i in bits 2(i%32)..2(i%32)+1 of 64-bit word i/32.
+ * Lowercase bases carry their code plus bit i%64 of lcmask word i/64; the
+ * reference gives them an all-zero one-hot row (sequence.cpp:312-319), so they
+ * contribute 0.  run_starts lists the stream positions where a new contiguous
+ * run begins (a new record or a gap of invalid bytes, SeqBlock::append,
+ * sequence.cpp:81-88); a window at q for a motif of length L is scored only if
+ * q + L <= the first run start after q (or nbases) (pwmscan.cpp:122-126).
+ */
+#ifndef BLAMM_HIP_H
+#define BLAMM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLAMM_HIP_ABI_VERSION 1
+
+/* Longest motif the engine scores (positions); longer motifs are rejected by
+ * blamm_hip_set_library with BLAMM_ERR_ARG. */
+#define BLAMM_MAX_MOTIF_LEN 32
+
+enum blamm_status {
+    BLAMM_OK = 0,
+    BLAMM_ERR_ARG = -1,       /* invalid argument */
+    BLAMM_ERR_DEVICE = -2,    /* HIP runtime error / no device */
+    BLAMM_ERR_NOMEM = -3,     /* device allocation failed */
+    BLAMM_ERR_OVERFLOW = -4,  /* hit capacity too small: nothing was dropped,
+                                 *nhits holds the required capacity */
+    BLAMM_ERR_STATE = -5      /* call out of order (e.g. scan before library) */
+};
+
+enum blamm_precision {
+    BLAMM_PREC_FP32 = 0,      /* FP32 MFMA, FP32 accumulate */
+    BLAMM_PREC_BF16X2 = 1     /* PWM split into bf16 hi+lo, BF16 MFMA, FP32 accumulate */
+};
+
+/* One occurrence.  pos: position in the packed stream of the scan call;
+ * col: PWM column in the caller's order; score: the window's PWM score.
+ * Replaces the (occIdx, occScore) pair of kernel.cu:30-31. */
+typedef struct blamm_hit {
+    uint64_t pos;
+    uint32_t col;
+    float score;
+} blamm_hit;
+
+/* Counters of the last completed scan on a context. */
+typedef struct blamm_scan_stats {
+    uint64_t candidates;      /* windows passing the MFMA filter band */
+    uint64_t band_rescored;   /* windows rescored in FP64 */
+    uint64_t hits;            /* occurrences reported */
+    double kernel_ms;         /* device time of the scan kernels (HIP events) */
+    double filter_ms;         /* of which: the MFMA filter kernels */
+} blamm_scan_stats;
+
+typedef struct blamm_hip_ctx blamm_hip_ctx;
+
+int blamm_hip_abi_version(void);
+
+/* Thread-local text of the last error on the calling thread ("" if none). */
+const char *blamm_hip_last_error(void);
+
+/* Number of visible HIP devices (replaces cudaGetDeviceCount, pwmscan.cpp:563). */
+int blamm_hip_device_count(int *count);
+
+/* Create / destroy a context on one device (replaces cudaSetDevice +
+ * cublasCreate, pwmscan.cpp:309-313, and the cudaFree/cublasDestroy teardown,
+ * pwmscan.cpp:428-436). */
+int blamm_hip_create(int device, blamm_hip_ctx **ctx);
+int blamm_hip_destroy(blamm_hip_ctx *ctx);
+
+/* Upload one group's pattern matrix and thresholds; they stay resident until
+ * replaced.  Replaces cudaMalloc+cublasSetVector of P and of the per-column
+ * thresholds (pwmscan.cpp:315-342).
+ *   P:          column-major, ldp >= 4*max(lengths); P[c*ldp + 4*p + b] is the
+ *               weight of base b (ACGT) at motif position p of column c
+ *               (MotifContainer::generateMatrix, motif.cpp:542-564)
+ *   lengths:    motif length of every column (1..BLAMM_MAX_MOTIF_LEN)
+ *   thresholds: per-column score threshold; a window is an occurrence iff its
+ *               score >= threshold (kernel.cu:29, pwmscan.cpp:117-119)
+ *   precision:  enum blamm_precision
+ * Every weight must be finite.  The hit set is that of a left-to-right FP64
+ * sum of the window's weights compared with (double)threshold in both
+ * precisions; the filter bands that make it so are per column and proven. */
+int blamm_hip_set_library(blamm_hip_ctx *ctx, const float *P, int ldp, int ncols,
+                          const int *lengths, const float *thresholds, int precision);
+
+/* Largest filter band (MFMA score vs FP64) and largest reported-score error
+ * bound over the resident library's columns. */
+int blamm_hip_library_bounds(blamm_hip_ctx *ctx, double *max_filter_band,
+                             double *max_score_error);
+
+/* Words a packed-stream / lowercase-mask buffer must hold for nbases (the
+ * engine reads a few words of zero padding past the end). */
+uint64_t blamm_hip_packed_words(uint64_t nbases);
+uint64_t blamm_hip_mask_words(uint64_t nbases);
+
+/* Reserve room for n filter candidates (device scratch).  Scans grow it on
+ * demand; reserving up front keeps device allocation out of later calls. */
+int blamm_hip_reserve(blamm_hip_ctx *ctx, uint64_t n);
+
+/* Scan host buffers.  packed has blamm_hip_packed_words(nbases) words; lcmask
+ * is NULL when the stream has no lowercase base, else it has
+ * blamm_hip_mask_words(nbases) words; run_starts are sorted.  Hits (unordered)
+ * go to hits[0..cap); *nhits receives their number.  If cap is too small the
+ * call returns BLAMM_ERR_OVERFLOW with *nhits = required capacity.
+ * nbases < 2^32 - 512 per call. */
+int blamm_hip_scan(blamm_hip_ctx *ctx, const uint64_t *packed, const uint64_t *lcmask,
+                   uint64_t nbases, const uint64_t *run_starts, uint64_t nruns,
+                   blamm_hit *hits, uint64_t cap, uint64_t *nhits);
+
+/* Enqueue a scan of device-resident buffers on `stream` (a hipStream_t, or
+ * NULL for the context's own stream) without synchronising.  Buffer sizes are
+ * passed so they can be checked.  The hit count is written to d_nhits
+ * (device uint64).  Call blamm_hip_scan_finish() to synchronise and to learn
+ * whether a capacity was exceeded. */
+int blamm_hip_scan_device(blamm_hip_ctx *ctx, const uint64_t *d_packed, uint64_t packed_words,
+                          const uint64_t *d_lcmask, uint64_t mask_words, uint64_t nbases,
+                          const uint64_t *d_run_starts, uint64_t nruns,
+                          blamm_hit *d_hits, uint64_t cap, uint64_t *d_nhits, void *stream);
+
+/* Synchronise the last scan_device; *nhits = hits produced (or required
+ * capacity on BLAMM_ERR_OVERFLOW, when the candidate scratch or the hit
+ * buffer was too small -- scratch is grown, so a retry succeeds). */
+int blamm_hip_scan_finish(blamm_hip_ctx *ctx, uint64_t *nhits);
+
+/* Counters of the last finished scan. */
+int blamm_hip_last_stats(blamm_hip_ctx *ctx, blamm_scan_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,166 @@
+"""ctypes binding of the oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this module, and only as the checker / the timed CPU baseline -- never as part
+of the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional, Tuple
+
+import numpy as np
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+BIN = os.path.join(ORACLE_DIR, "_build", "blamm_oracle")
+
+HIT_DTYPE = np.dtype([("pos", "<u8"), ("col", "<u4"), ("pad", "<u4"), ("score64", "<f8"),
+                      ("score32", "<f4"), ("pad2", "<f4")])
+
+_lib = None
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int)
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.oracle_log2f.argtypes = [ctypes.c_float]
+        L.oracle_log2f.restype = ctypes.c_float
+        L.oracle_pfm2pwm.argtypes = [_u64p, ctypes.c_int, _u64p, ctypes.c_float, ctypes.c_int, _f32p]
+        L.oracle_max_score.argtypes = [_f32p, ctypes.c_int]
+        L.oracle_max_score.restype = ctypes.c_float
+        L.oracle_min_score.argtypes = [_f32p, ctypes.c_int]
+        L.oracle_min_score.restype = ctypes.c_float
+        L.oracle_score_cutoff.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.oracle_score_cutoff.restype = ctypes.c_float
+        L.oracle_revcompl_pfm.argtypes = [_u64p, ctypes.c_int]
+        L.oracle_scan_fp64.argtypes = [ctypes.c_char_p, ctypes.c_uint64, _u64p, ctypes.c_uint64, _f32p,
+                                       ctypes.c_int, _i32p, _f32p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_void_p)]
+        L.oracle_scan_fp64.restype = ctypes.c_int64
+        L.oracle_scan_sgemm.argtypes = [ctypes.c_char_p, ctypes.c_uint64, _u64p, ctypes.c_uint64, _f32p,
+                                        ctypes.c_int, _i32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.oracle_scan_sgemm.restype = ctypes.c_int64
+        L.oracle_blas_open.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.oracle_free.argtypes = [ctypes.c_void_p]
+        L.oracle_theoretical_hist.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_size_t, ctypes.c_size_t,
+                                              _u64p, _f32p, _f32p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def log2f(x: float) -> float:
+    return lib().oracle_log2f(x)
+
+
+def pfm2pwm(pfm: np.ndarray, bg, pseudocount: float = 0.25, rc: bool = False) -> np.ndarray:
+    pfm = np.ascontiguousarray(pfm, dtype=np.uint64)
+    bg = np.ascontiguousarray(bg, dtype=np.uint64)
+    out = np.zeros((pfm.shape[0], 4), np.float32)
+    lib().oracle_pfm2pwm(_p(pfm, ctypes.c_uint64), pfm.shape[0], _p(bg, ctypes.c_uint64),
+                         ctypes.c_float(pseudocount), int(rc), _p(out, ctypes.c_float))
+    return out
+
+
+def max_score(w: np.ndarray) -> float:
+    w = np.ascontiguousarray(w, np.float32)
+    return lib().oracle_max_score(_p(w, ctypes.c_float), w.shape[0])
+
+
+def min_score(w: np.ndarray) -> float:
+    w = np.ascontiguousarray(w, np.float32)
+    return lib().oracle_min_score(_p(w, ctypes.c_float), w.shape[0])
+
+
+def score_cutoff(counts, min_score_: float, max_score_: float, pvalue: float) -> float:
+    c = np.ascontiguousarray(counts, np.uint64)
+    return lib().oracle_score_cutoff(_p(c, ctypes.c_uint64), c.shape[0], ctypes.c_float(min_score_),
+                                     ctypes.c_float(max_score_), ctypes.c_float(pvalue))
+
+
+def theoretical_hist(w: np.ndarray, bg_prob, num_bins: int = 250, max_length: int = 10_000_000):
+    w = np.ascontiguousarray(w, np.float32)
+    bg = np.ascontiguousarray(bg_prob, np.float32)
+    counts = np.zeros(num_bins, np.uint64)
+    mn, mx = ctypes.c_float(), ctypes.c_float()
+    lib().oracle_theoretical_hist(_p(w, ctypes.c_float), w.shape[0], _p(bg, ctypes.c_float), num_bins,
+                                  max_length, _p(counts, ctypes.c_uint64), ctypes.byref(mn), ctypes.byref(mx))
+    return counts, mn.value, mx.value
+
+
+def _stack(pwms):
+    lmax = max(w.shape[0] for w in pwms)
+    P = np.zeros((len(pwms), lmax, 4), np.float32)
+    for c, w in enumerate(pwms):
+        P[c, :w.shape[0]] = w
+    return P, np.array([w.shape[0] for w in pwms], np.int32), lmax
+
+
+def _hits(ptr: ctypes.c_void_p, n: int) -> np.ndarray:
+    if n <= 0:
+        if ptr:
+            lib().oracle_free(ptr)
+        return np.zeros(0, HIT_DTYPE)
+    buf = (ctypes.c_char * (n * HIT_DTYPE.itemsize)).from_address(ptr.value)
+    out = np.frombuffer(bytes(buf), dtype=HIT_DTYPE).copy()
+    lib().oracle_free(ptr)
+    return out
+
+
+def scan_fp64(stream: bytes, run_starts, pwms, thresholds, threads: int = 8) -> np.ndarray:
+    """Brute-force FP64 scan of a filtered stream (bytes ACGTacgt)."""
+    P, lens, lmax = _stack(pwms)
+    thr = np.ascontiguousarray(thresholds, np.float32)
+    rs = np.ascontiguousarray(run_starts if run_starts is not None and len(run_starts) else [0], np.uint64)
+    out = ctypes.c_void_p()
+    n = lib().oracle_scan_fp64(stream, len(stream), _p(rs, ctypes.c_uint64), len(rs), _p(P, ctypes.c_float), lmax,
+                               _p(lens, ctypes.c_int), _p(thr, ctypes.c_float), len(pwms), threads,
+                               ctypes.byref(out))
+    if n < 0:
+        raise RuntimeError("oracle scan failed")
+    return _hits(out, n)
+
+
+def blas_open(path: str = "") -> None:
+    err = ctypes.create_string_buffer(1024)
+    if lib().oracle_blas_open(path.encode(), err, 1024) != 0:
+        raise RuntimeError(err.value.decode())
+
+
+def scan_sgemm(stream: bytes, run_starts, pwms, thresholds, h: int = 1000, w: int = 250,
+               min_zero_area: int = 4096, threads: int = 8) -> np.ndarray:
+    """The reference's sgemm scan restated (columns sorted by length)."""
+    blas_open()
+    P, lens, lmax = _stack(pwms)
+    thr = np.ascontiguousarray(thresholds, np.float32)
+    rs = np.ascontiguousarray(run_starts if run_starts is not None and len(run_starts) else [0], np.uint64)
+    out = ctypes.c_void_p()
+    n = lib().oracle_scan_sgemm(stream, len(stream), _p(rs, ctypes.c_uint64), len(rs), _p(P, ctypes.c_float), lmax,
+                                _p(lens, ctypes.c_int), _p(thr, ctypes.c_float), len(pwms), h, w, min_zero_area,
+                                threads, ctypes.byref(out))
+    if n < 0:
+        raise RuntimeError(f"oracle sgemm scan failed ({n})")
+    return _hits(out, n)
+
+
+def run_cli(args, cwd: str, check: bool = True) -> subprocess.CompletedProcess:
+    if not os.path.exists(BIN):
+        build()
+    return subprocess.run([BIN] + list(args), cwd=cwd, capture_output=True, text=True, check=check)
