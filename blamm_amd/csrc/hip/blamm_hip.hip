@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -107,6 +108,7 @@ double gamma_n(double n, double u) { return n * u / (1.0 - n * u); }
 struct blamm_hip_ctx {
     int device = 0;
     int num_cu = 256;
+    int waves_per_cu = 16;      // launch size: waves per CU per class launch
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
@@ -183,9 +185,9 @@ int blamm_hip_abi_version(void) { return BLAMM_HIP_ABI_VERSION; }
 
 const char* blamm_hip_last_error(void) { return g_last_error.c_str(); }
 
-uint64_t blamm_hip_packed_words(uint64_t nbases) { return (nbases + 31) / 32 + 8; }
+uint64_t blamm_hip_packed_words(uint64_t nbases) { return (nbases + 31) / 32 + 16; }
 
-uint64_t blamm_hip_mask_words(uint64_t nbases) { return (nbases + 63) / 64 + 8; }
+uint64_t blamm_hip_mask_words(uint64_t nbases) { return (nbases + 63) / 64 + 16; }
 
 int blamm_hip_device_count(int* count)
 {
@@ -214,6 +216,10 @@ int blamm_hip_create(int device, blamm_hip_ctx** out)
     blamm_hip_ctx* c = new blamm_hip_ctx();
     c->device = device;
     c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (const char* e = std::getenv("BLAMM_WAVES_PER_CU")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= 1024) c->waves_per_cu = v;
+    }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = c->counters.reserve(kCntSlots);
@@ -477,8 +483,7 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[0], s));
     if (nbases > 0) {
-        const uint32_t tile = bf ? kBf16Tile : kFp32Tile;
-        const uint32_t ntiles = (uint32_t)((nbases + tile - 1) / tile);
+        const uint32_t ntiles = (uint32_t)((nbases + kBf16Tile - 1) / kBf16Tile);   // 32-window tiles
         const int nb = bf ? kNbBf16 : kNbFp32;
         for (const ClassInfo& ci : c->classes) {
             FilterArgs a;
@@ -487,7 +492,7 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
             a.nbases = (uint32_t)nbases;
             a.ntiles = ntiles;
             a.ngroups = (uint32_t)(ci.nblk / nb);
-            const uint64_t target = (uint64_t)c->num_cu * 16;
+            const uint64_t target = (uint64_t)c->num_cu * (uint64_t)c->waves_per_cu;
             uint64_t ranges = (target + a.ngroups - 1) / a.ngroups;
             ranges = std::max<uint64_t>(1, std::min<uint64_t>(ranges, ntiles));
             a.tiles_per_wave = (uint32_t)((ntiles + ranges - 1) / ranges);

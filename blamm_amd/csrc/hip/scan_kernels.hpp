@@ -79,24 +79,73 @@ struct FilterArgs {
     uint64_t cand_cap;
 };
 
+// Per-wave staging of filter candidates in LDS.  A single global counter hit
+// by every candidate serialises on one L2 atomic unit (tens of millions of
+// candidates per scan), so each wave collects up to kStage candidates in its
+// own LDS slice and reserves global space with one atomic per flush.  Only
+// the owning wave touches its slice; LDS operations of one wave complete in
+// program order, so no barrier is needed.
+constexpr int kStage = 512;
+
+struct Stage {
+    uint2* buf;        // this wave's LDS slice
+    uint32_t n;        // staged entries (wave-uniform)
+};
+
+__device__ __forceinline__ void stage_flush(const FilterArgs& a, Stage& st)
+{
+    if (st.n == 0) return;
+    const uint32_t lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&a.counters[kCntCand], (unsigned long long)st.n);
+    base = __shfl(base, 0);
+    for (uint32_t i = lane; i < st.n; i += 64) {
+        const uint64_t at = (uint64_t)base + i;
+        if (at < a.cand_cap) a.cand[at] = st.buf[i];
+    }
+    st.n = 0;
+}
+
 // Emit (window, slot) for every accumulator entry clearing its threshold.
-// Called only when the wave holds at least one such entry.
+// Called only when the wave holds at least one such entry.  `full` (wave
+// uniform) says every window of the tile lies inside the stream.
 template <int NREG>
-__device__ __forceinline__ void emit_candidates(const FilterArgs& a, const float* acc, float thr,
+__device__ __forceinline__ void emit_candidates(const FilterArgs& a, Stage& st, const float* acc, float thr,
                                                 uint32_t q_of_reg0, const uint32_t* row_of_reg,
-                                                uint32_t slot)
+                                                uint32_t slot, bool full)
 {
 #pragma unroll
     for (int r = 0; r < NREG; ++r) {
         const uint32_t q = q_of_reg0 + row_of_reg[r];
-        const bool pred = (acc[r] >= thr) && (q < a.nbases);
+        const bool pred = (acc[r] >= thr) && (full || q < a.nbases);
         const uint64_t m = __ballot(pred);
         if (m) {
-            const uint64_t at = wave_reserve(&a.counters[kCntCand], m);
-            if (pred && at < a.cand_cap) a.cand[at] = make_uint2(q, slot);
+            if (st.n > (uint32_t)(kStage - 64)) stage_flush(a, st);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (pred) st.buf[st.n + rank] = make_uint2(q, slot);
+            st.n += (uint32_t)__popcll(m);
         }
     }
 }
+
+// 64 bits of the 128-bit value (hi:lo) starting at bit s (0 <= s < 64),
+// without a branch for s == 0.
+__device__ __forceinline__ uint64_t funnel64(uint64_t lo, uint64_t hi, uint32_t s, uint32_t s_inv)
+{
+    return (lo >> s) | ((hi << 1) << s_inv);   // s_inv = 63 - s
+}
+
+__device__ __forceinline__ uint32_t funnel32(uint32_t lo, uint32_t hi, uint32_t s)
+{
+    return __builtin_amdgcn_alignbit(hi, lo, s);
+}
+
+// Tiles are 32 consecutive windows; a wave walks tiles [t0, t1) of its
+// range.  Lane-private bases for tile t come from packed words t+woff and
+// t+woff+1 (a sliding pair), so the loop keeps a register window of kGroup+1
+// words and prefetches the next group's words one group ahead.
+constexpr int kGroup = 4;
 
 // ---------------------------------------------------------------------------
 // Split-BF16 filter: W = hi + lo (bf16 each); one-hot operand is exact in bf16.
@@ -108,11 +157,55 @@ __device__ __forceinline__ void emit_candidates(const FilterArgs& a, const float
 // streams window tiles through them.
 // ---------------------------------------------------------------------------
 template <int NCH, int NB, bool MASK>
+__device__ __forceinline__ void bf16_tile(const FilterArgs& a, Stage& st, const bf16x8 (&bh)[NB][NCH],
+                                          const bf16x8 (&bl)[NB][NCH], const float (&th)[NB],
+                                          const uint32_t (&rows)[16], uint32_t q0, uint64_t bits,
+                                          uint32_t mb, uint32_t blk0, uint32_t lane)
+{
+    f32x16 acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
+#pragma unroll
+    for (int j = 0; j < NCH; ++j) {
+        const uint32_t b0 = (uint32_t)(bits >> (8 * j)) & 3u;
+        const uint32_t b1 = (uint32_t)(bits >> (8 * j + 2)) & 3u;
+        uint64_t o0 = 0x3F80ull << (b0 * 16);   // bf16 1.0 in lane b0 of 4
+        uint64_t o1 = 0x3F80ull << (b1 * 16);
+        if (MASK) {
+            if ((mb >> (4 * j)) & 1) o0 = 0;
+            if ((mb >> (4 * j + 1)) & 1) o1 = 0;
+        }
+        const u32x4 av = {(uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32)};
+        const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bh[b][j], acc[b], 0, 0, 0);
+            acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bl[b][j], acc[b], 0, 0, 0);
+        }
+    }
+    const bool full = q0 + kBf16Tile <= a.nbases;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        float mx = acc[b][0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[b][r]);
+        if (__ballot(mx >= th[b])) {
+            float v[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = acc[b][r];
+            emit_candidates<16>(a, st, v, th[b], q0, rows, (blk0 + b) * kBf16Cols + (lane & 31), full);
+        }
+    }
+}
+
+template <int NCH, int NB, bool MASK>
 __global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (wave >= a.nwaves) return;
+    __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
+    Stage st{stage_lds[threadIdx.x >> 6], 0u};
     const uint32_t g = wave % a.ngroups;
     const uint32_t rng = wave / a.ngroups;
     const uint32_t t0 = rng * a.tiles_per_wave;
@@ -140,60 +233,92 @@ __global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
 #pragma unroll
     for (int r = 0; r < 16; ++r) rows[r] = 8 * (r / 4) + 4 * hh + (r % 4);
 
-    for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t q0 = t * kBf16Tile;
-        const uint64_t start = (uint64_t)q0 + m + 2 * hh;
-        const uint64_t bits = load_bits64(a.packed, 2 * start);
-        uint64_t mb = 0;
-        if (MASK) mb = load_bits64(a.lcmask, start);
-        f32x16 acc[NB];
+    const uint32_t off = m + 2 * hh;                 // first base of the lane's window, 0..33
+    const uint32_t woff = off >> 5;
+    const uint32_t sh = 2 * (off & 31), sh_inv = 63 - sh;
+    const uint64_t* __restrict__ pk = a.packed + woff;
+    const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + woff;
+    uint64_t w[kGroup + 1];
+    uint32_t mw[kGroup + 1];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
+    for (int u = 0; u <= kGroup; ++u) {
+        w[u] = pk[t0 + u];
+        mw[u] = MASK ? mk[t0 + u] : 0u;
+    }
+    for (uint32_t t = t0; t < t1; t += kGroup) {
+        uint64_t nx[kGroup];
+        uint32_t nm[kGroup];
+        if (t + kGroup < t1) {
 #pragma unroll
-        for (int j = 0; j < NCH; ++j) {
-            const uint32_t b0 = (uint32_t)(bits >> (8 * j)) & 3u;
-            const uint32_t b1 = (uint32_t)(bits >> (8 * j + 2)) & 3u;
-            uint64_t o0 = 0x3F80ull << (b0 * 16);   // bf16 1.0 in lane b0 of 4
-            uint64_t o1 = 0x3F80ull << (b1 * 16);
-            if (MASK) {
-                if ((mb >> (4 * j)) & 1) o0 = 0;
-                if ((mb >> (4 * j + 1)) & 1) o1 = 0;
-            }
-            const u32x4 av = {(uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32)};
-            const bf16x8 A = __builtin_bit_cast(bf16x8, av);
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bh[b][j], acc[b], 0, 0, 0);
-                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bl[b][j], acc[b], 0, 0, 0);
+            for (int u = 0; u < kGroup; ++u) {
+                nx[u] = pk[t + kGroup + 1 + u];
+                nm[u] = MASK ? mk[t + kGroup + 1 + u] : 0u;
             }
         }
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            float mx = acc[b][0];
-#pragma unroll
-            for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[b][r]);
-            if (__ballot(mx >= th[b])) {
-                float v[16];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) v[r] = acc[b][r];
-                emit_candidates<16>(a, v, th[b], q0, rows, (blk0 + b) * kBf16Cols + (lane & 31));
+        for (int u = 0; u < kGroup; ++u) {
+            if (t + u < t1) {
+                const uint64_t bits = funnel64(w[u], w[u + 1], sh, sh_inv);
+                const uint32_t mb = MASK ? funnel32(mw[u], mw[u + 1], off & 31) : 0u;
+                bf16_tile<NCH, NB, MASK>(a, st, bh, bl, th, rows, (t + u) * kBf16Tile, bits, mb, blk0, lane);
             }
+        }
+        w[0] = w[kGroup];
+        mw[0] = mw[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+            w[u + 1] = nx[u];
+            mw[u + 1] = nm[u];
         }
     }
+    stage_flush(a, st);
 }
 
 // ---------------------------------------------------------------------------
 // FP32 filter: v_mfma_f32_16x16x4_f32, one motif position per MFMA.
 // A = one-hot (16 windows x 4 bases), B = W_c[p][.] (4 x 16 columns).
 // Lane l: A[l%16][l/16], B[l/16][l%16]; D lane l holds column l%16,
-// rows 4(l/16) + r.
+// rows 4(l/16) + r.  A 32-window tile is two 16-window sub-tiles.
 // ---------------------------------------------------------------------------
+template <int NCH, int NB, bool MASK>
+__device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, const float (&bw)[NB][4 * NCH],
+                                             const float (&th)[NB], const uint32_t (&rows)[4], uint32_t q0,
+                                             uint64_t bits, uint32_t mb, uint32_t k, uint32_t blk0,
+                                             uint32_t lane)
+{
+    constexpr int NP = 4 * NCH;
+    f32x4 acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const uint32_t base = (uint32_t)(bits >> (2 * p)) & 3u;
+        bool on = (base == k);
+        if (MASK) on = on && !((mb >> p) & 1);
+        const float av = on ? 1.0f : 0.0f;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bw[b][p], acc[b], 0, 0, 0);
+    }
+    const bool full = q0 + kFp32Tile <= a.nbases;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const float mx = fmaxf(fmaxf(acc[b][0], acc[b][1]), fmaxf(acc[b][2], acc[b][3]));
+        if (__ballot(mx >= th[b])) {
+            float v[4] = {acc[b][0], acc[b][1], acc[b][2], acc[b][3]};
+            emit_candidates<4>(a, st, v, th[b], q0, rows, (blk0 + b) * kFp32Cols + (lane & 15), full);
+        }
+    }
+}
+
 template <int NCH, int NB, bool MASK>
 __global__ __launch_bounds__(kThreads) void filter_fp32_kernel(FilterArgs a)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (wave >= a.nwaves) return;
+    __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
+    Stage st{stage_lds[threadIdx.x >> 6], 0u};
     const uint32_t g = wave % a.ngroups;
     const uint32_t rng = wave / a.ngroups;
     const uint32_t t0 = rng * a.tiles_per_wave;
@@ -215,35 +340,47 @@ __global__ __launch_bounds__(kThreads) void filter_fp32_kernel(FilterArgs a)
     uint32_t rows[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) rows[r] = 4 * k + r;
-
-    for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t q0 = t * kFp32Tile;
-        const uint64_t start = (uint64_t)q0 + i;
-        const uint64_t bits = load_bits64(a.packed, 2 * start);
-        uint64_t mb = 0;
-        if (MASK) mb = load_bits64(a.lcmask, start);
-        f32x4 acc[NB];
+    const uint32_t s0 = 2 * i, s1 = 2 * (i + 16);    // bit offsets of the two sub-tiles
+    const uint64_t* __restrict__ pk = a.packed;
+    const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask;
+    uint64_t w[kGroup + 1];
+    uint32_t mw[kGroup + 1];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u <= kGroup; ++u) {
+        w[u] = pk[t0 + u];
+        mw[u] = MASK ? mk[t0 + u] : 0u;
+    }
+    for (uint32_t t = t0; t < t1; t += kGroup) {
+        uint64_t nx[kGroup];
+        uint32_t nm[kGroup];
+        if (t + kGroup < t1) {
 #pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const uint32_t base = (uint32_t)(bits >> (2 * p)) & 3u;
-            bool on = (base == k);
-            if (MASK) on = on && !((mb >> p) & 1);
-            const float av = on ? 1.0f : 0.0f;
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-                acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bw[b][p], acc[b], 0, 0, 0);
-        }
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const float mx = fmaxf(fmaxf(acc[b][0], acc[b][1]), fmaxf(acc[b][2], acc[b][3]));
-            if (__ballot(mx >= th[b])) {
-                float v[4] = {acc[b][0], acc[b][1], acc[b][2], acc[b][3]};
-                emit_candidates<4>(a, v, th[b], q0, rows, (blk0 + b) * kFp32Cols + (lane & 15));
+            for (int u = 0; u < kGroup; ++u) {
+                nx[u] = pk[t + kGroup + 1 + u];
+                nm[u] = MASK ? mk[t + kGroup + 1 + u] : 0u;
             }
         }
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+            if (t + u < t1) {
+                const uint32_t q0 = (t + u) * kBf16Tile;
+                const uint64_t b0 = funnel64(w[u], w[u + 1], s0, 63 - s0);
+                const uint64_t b1 = funnel64(w[u], w[u + 1], s1, 63 - s1);
+                const uint32_t m0 = MASK ? funnel32(mw[u], mw[u + 1], i) : 0u;
+                const uint32_t m1 = MASK ? funnel32(mw[u], mw[u + 1], i + 16) : 0u;
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, rows, q0, b0, m0, k, blk0, lane);
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, rows, q0 + kFp32Tile, b1, m1, k, blk0, lane);
+            }
+        }
+        w[0] = w[kGroup];
+        mw[0] = mw[kGroup];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+            w[u + 1] = nx[u];
+            mw[u + 1] = nm[u];
+        }
     }
+    stage_flush(a, st);
 }
 
 struct FinalizeArgs {
@@ -272,6 +409,8 @@ struct DevHit {
     float score;
 };
 
+constexpr int kHitStage = 256;
+
 // Exact decision for each candidate: run-boundary rule (pwmscan.cpp:122-126),
 // then the left-to-right FP32 sum; windows whose FP32 sum lies within its
 // proven band of the threshold are rescored in FP64 (the only FP64 step).
@@ -283,6 +422,19 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
     const uint64_t wave = (uint64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
     DevHit* hits = (DevHit*)a.hits;
+    // hits are staged per wave in LDS and reserved with one atomic per flush
+    __shared__ DevHit hit_lds[kWavesPerBlock][kHitStage];
+    DevHit* sbuf = hit_lds[threadIdx.x >> 6];
+    uint32_t sn = 0;
+    auto flush = [&]() {
+        if (sn == 0) return;
+        unsigned long long at0 = 0;
+        if (lane == 0) at0 = atomicAdd(&a.counters_rw[kCntHits], (unsigned long long)sn);
+        at0 = __shfl(at0, 0);
+        for (uint32_t i = lane; i < sn; i += 64)
+            if ((uint64_t)at0 + i < a.hit_cap) hits[at0 + i] = sbuf[i];
+        sn = 0;
+    };
     for (uint64_t base = wave * 64; base < ncand; base += nw * 64) {
         const uint64_t idx = base + lane;
         bool hit = false;
@@ -327,16 +479,20 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
         }
         const uint64_t m = __ballot(hit);
         if (m) {
-            const uint64_t at = wave_reserve(&a.counters_rw[kCntHits], m);
-            if (hit && at < a.hit_cap) {
+            if (sn > (uint32_t)(kHitStage - 64)) flush();
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (hit) {
                 DevHit h;
                 h.pos = q;
                 h.col = a.slot_col[slot];
                 h.score = score;
-                hits[at] = h;
+                sbuf[sn + rank] = h;
             }
+            sn += (uint32_t)__popcll(m);
         }
     }
+    flush();
 }
 
 __global__ void zero_counters_kernel(unsigned long long* c)
