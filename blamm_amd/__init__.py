@@ -10,6 +10,7 @@ when the native libraries are missing.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
 from typing import Optional, Tuple
 
@@ -63,11 +64,21 @@ def _need(path: str) -> None:
                            f"(make -C blamm_amd/csrc) first")
 
 
+def _share_torch_hip_runtime() -> None:
+    """A process must hold a single HIP runtime.  torch wheels bundle their own
+    libamdhip64 (soname libamdhip64.so.7); importing torch first makes
+    libblamm_hip.so bind to that copy instead of loading /opt/rocm's beside it,
+    after which torch would find no GPU."""
+    if importlib.util.find_spec("torch") is not None:
+        import torch  # noqa: F401
+
+
 def hip_lib() -> ctypes.CDLL:
     """The HIP engine (fails loudly when it is not built)."""
     global _hip
     if _hip is None:
         _need(HIP_LIB)
+        _share_torch_hip_runtime()
         lib = ctypes.CDLL(HIP_LIB)
         vp = ctypes.c_void_p
         lib.blamm_hip_last_error.restype = ctypes.c_char_p

@@ -353,13 +353,18 @@ int scan_main(int argc, char** argv)
     else if (relSpec) std::cout << "Relative motif score threshold set to: " << relTh << std::endl;
     else std::cout << "P-value motif score threshold set to: " << pvalue << std::endl;
 
-    int ndev = 0;
-    blamm_hip_device_count(&ndev);
-    if (ndev == 0) throw std::runtime_error("HIP error: no devices found. Aborting...");
-    std::cout << "Using 1 GPU device (" << (precision == BLAMM_PREC_FP32 ? "fp32" : "split-bf16")
-              << " MFMA) and " << numThreads << " host thread(s)" << std::endl;
+    // The device is opened when the first group is ready to scan, so input
+    // errors (dict, motifs, histograms) are reported first.
     HipContext hc;
-    hip_check(blamm_hip_create(device, &hc.ctx), "device setup");
+    auto open_device = [&]() {
+        if (hc.ctx) return;
+        int ndev = 0;
+        blamm_hip_device_count(&ndev);
+        if (ndev == 0) throw std::runtime_error("HIP error: no devices found. Aborting...");
+        std::cout << "Using 1 GPU device (" << (precision == BLAMM_PREC_FP32 ? "fp32" : "split-bf16")
+                  << " MFMA) and " << numThreads << " host thread(s)" << std::endl;
+        hip_check(blamm_hip_create(device, &hc.ctx), "device setup");
+    };
 
     const size_t ncols = motifs.size();
     std::vector<ColumnInfo> cols(ncols);
@@ -407,6 +412,7 @@ int scan_main(int argc, char** argv)
                     << "\t" << max_score(w, motifs[c].size()) << std::endl;
         }
         const PackedStream s = pack_fasta(g.files, g.totSeqLen);
+        open_device();
         hip_check(blamm_hip_set_library(hc.ctx, P.data(), (int)ldp, (int)ncols, lengths.data(), thr.data(), precision),
                   "library upload");
         // Segments of the stream (multiples of 64 bases) with a halo of

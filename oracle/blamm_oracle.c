@@ -25,6 +25,10 @@
  *                         background) and the store-not-add bin quirk
  *                         (motif.h:109-115) on a constructed motif
  *   dict writer           closed-form ACGT counts of hand-built FASTA
+ * Planted mistakes: compiling with -DORACLE_MUTATE_<name> swaps one rule for a
+ * plausible wrong one (GT, NO_BG_SWAP, LOWER_COUNTS, NO_RUN_CHECK,
+ * CUTOFF_DOUBLE, HIST_ADD, RUNS_IGNORE_RECORD); tests/test_oracle_mutations.py
+ * shows each is caught by a pinning check.
  * The reference itself is not built here: it needs the CMake-generated
  * config.h and an external BLAS (see the design notes), so no run of the
  * reference binary pins any of the above.
@@ -217,7 +221,11 @@ void oracle_pfm2pwm(const uint64_t *pfm, int L, const uint64_t *bg,
     bgTot += 4.0f * pc;
     float bgProb[4];
     for (int i = 0; i < 4; i++) bgProb[i] = ((float)bg[i] + pc) / bgTot;
+#ifdef ORACLE_MUTATE_NO_BG_SWAP
+    if (0) {
+#else
     if (rc) {   /* motif.cpp:204-207 */
+#endif
         float t = bgProb[0]; bgProb[0] = bgProb[3]; bgProb[3] = t;
         t = bgProb[1]; bgProb[1] = bgProb[2]; bgProb[2] = t;
     }
@@ -445,8 +453,13 @@ float oracle_score_cutoff(const uint64_t *counts, size_t numBins, float minScore
         if ((double)counts[i] < curr) curr -= (double)counts[i];
         else {
             double frac = curr / (double)counts[i];
+#ifdef ORACLE_MUTATE_CUTOFF_DOUBLE
+            double cutoffi = frac * (double)i + (1.0 - frac) * (double)(i + 1);
+            return (float)(cutoffi * (double)width + (double)minScore);
+#else
             float cutoffi = (float)(frac * (double)i + (1.0 - frac) * (double)(i + 1));
             return cutoffi * width + minScore;
+#endif
         }
     }
     return maxScore;
@@ -551,7 +564,11 @@ int oracle_theoretical_hist(const float *pwm, int L, const float *bg,
         int bin = (int)((score - minS) / width);
         if (bin < 0) bin = 0;
         if (bin > (int)numBins - 1) bin = (int)numBins - 1;
+#ifdef ORACLE_MUTATE_HIST_ADD
+        counts[bin] += cnt;
+#else
         counts[bin] = cnt;   /* store, not add (motif.h:109-115) */
+#endif
     }
     free(cur); free(nxt); free(cp); free(np); free(pi);
     *minScoreOut = minS; *maxScoreOut = maxS;
@@ -615,7 +632,11 @@ int64_t oracle_filter_fasta(const char *const *files, int nfiles, uint64_t maxLe
                 /* appendNextBlock truncation at maxFiltSeqLen (sequence.cpp:219-220) */
                 if (n + pieceLen > maxLen) { pieceLen = maxLen - n; done = 1; }
                 if (pieceLen == 0) break;
+#ifdef ORACLE_MUTATE_RUNS_IGNORE_RECORD
+                int contiguous = havePiece && lastEndPos == piecePos;
+#else
                 int contiguous = havePiece && lastSeq == seqIdx && lastEndPos == piecePos;
+#endif
                 if (!contiguous) {
                     if (nr == rcap) { rcap *= 2; rs = xrealloc(rs, rcap * 8); rq = xrealloc(rq, rcap * 8); rp = xrealloc(rp, rcap * 8); }
                     rs[nr] = n; rq[nr] = seqIdx; rp[nr] = piecePos; nr++;
@@ -644,6 +665,9 @@ static int base_index(int ch)
     /* one-hot row of SeqMatrix::getNextSeqMatrix (sequence.cpp:312-319):
      * uppercase only; lowercase is a valid base with an all-zero row */
     switch (ch) { case 'A': return 0; case 'C': return 1; case 'G': return 2; case 'T': return 3; }
+#ifdef ORACLE_MUTATE_LOWER_COUNTS
+    switch (ch) { case 'a': return 0; case 'c': return 1; case 'g': return 2; case 't': return 3; }
+#endif
     return 4;
 }
 
@@ -680,14 +704,22 @@ static void *fp64_worker(void *arg)
         uint64_t rem = next_break(j->runStart, j->nruns, q, j->n) - q;
         for (int c = 0; c < j->ncols; c++) {
             int L = j->len[c];
+#ifdef ORACLE_MUTATE_NO_RUN_CHECK
+            if ((uint64_t)L > j->n - q) continue;
+#else
             if ((uint64_t)L > rem) continue;          /* pwmscan.cpp:122-126 */
+#endif
             const float *w = j->pwm + (size_t)c * (size_t)j->ldw * 4;
             double s = 0.0;
             for (int p = 0; p < L; p++) {
                 int b = base_index((unsigned char)j->stream[q + (uint64_t)p]);
                 if (b < 4) s += (double)w[4 * p + b];
             }
+#ifdef ORACLE_MUTATE_GT
+            if (s > (double)j->thr[c])
+#else
             if (s >= (double)j->thr[c])                /* pwmscan.cpp:117-119 */
+#endif
                 push_hit(&j->hits, &j->nh, &j->cap, q, (uint32_t)c, s, (float)s);
         }
     }

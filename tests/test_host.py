@@ -1,0 +1,171 @@
+"""Host side of the product against the oracle (CPU only)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import blamm_amd as ba
+from blamm_amd import synth
+from helpers import EXAMPLE, copy_example, run_blamm, run_oracle
+
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+
+
+def declared(header):
+    text = open(os.path.join(INCLUDE, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(blamm_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("blamm_hip.h", ba.HIP_LIB), ("blamm_host.h", ba.HOST_LIB)])
+def test_library_exports_every_declared_symbol(header, lib):
+    names = declared(header)
+    assert len(names) >= 5
+    so = ctypes.CDLL(lib)
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, missing
+
+
+def test_hip_library_reports_no_device_without_gpu():
+    """On a host without a GPU the engine refuses to create a context (no CPU
+    fallback exists)."""
+    n = ctypes.c_int(-1)
+    lib = ba.hip_lib()
+    assert lib.blamm_hip_device_count(ctypes.byref(n)) == 0
+    if n.value > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(ba.BlammError) as e:
+        ba.HipScanner(0)
+    assert e.value.code == ba.BLAMM_ERR_DEVICE
+
+
+def _write_synthetic(d, seed=3, lower=True, crlf=False, groups=2):
+    rng = np.random.default_rng(seed)
+    files = []
+    for g in range(groups):
+        recs = []
+        for r in range(3):
+            n = int(rng.integers(500, 3000))
+            recs.append(synth.Record(f"g{g}r{r}", synth.bases(n, 0.3 + 0.1 * g, rng), synth.n_runs(n, 0.02, 5, 40, rng)))
+        fn = os.path.join(d, f"g{g}.fa")
+        synth.write_fasta(fn, recs, width=57)
+        data = open(fn, "rb").read()
+        if lower:
+            data = bytes(c | 0x20 if (i % 7 == 3 and chr(c) in "ACGT") else c for i, c in enumerate(data))
+        if crlf:
+            data = data.replace(b"\n", b"\r\n")
+        open(fn, "wb").write(data)
+        files.append((f"grp{g}", fn))
+    with open(os.path.join(d, "sequences.mf"), "w") as f:
+        for g, fn in files:
+            f.write(f"{g}\t{fn}\n")
+    ms = synth.motifs(6, 3, 14, 0.5, seed=seed)
+    ms.write_jaspar(os.path.join(d, "motifs.jaspar"))
+
+
+@pytest.mark.parametrize("variant", ["example", "lower", "crlf"])
+def test_dict_and_hist_match_oracle(tmp_path, variant):
+    a, b = tmp_path / "prod", tmp_path / "orc"
+    for d in (a, b):
+        d.mkdir()
+        if variant == "example":
+            copy_example(str(d))
+        else:
+            _write_synthetic(str(d), lower=True, crlf=variant == "crlf")
+    run_blamm(["dict", "sequences.mf"], str(a))
+    run_oracle(["dict", "sequences.mf"], str(b))
+    da, db = open(a / "sequences.mf.dict").read(), open(b / "sequences.mf.dict").read()
+    if variant != "example":   # synthetic files live at absolute tmp paths
+        da, db = da.replace(str(a), "X"), db.replace(str(b), "X")
+    assert da == db
+    for extra in ([], ["-b", "97", "-l", "123456"]):
+        run_blamm(["hist"] + extra + ["motifs.jaspar", "sequences.mf"], str(a))
+        run_oracle(["hist"] + extra + ["motifs.jaspar", "sequences.mf"], str(b))
+        dats = sorted(p.name for p in a.glob("hist_*.dat"))
+        assert dats and dats == sorted(p.name for p in b.glob("hist_*.dat"))
+        for name in dats:
+            assert open(a / name).read() == open(b / name).read(), name
+
+
+def test_pwm_matches_oracle(orc):
+    rng = np.random.default_rng(17)
+    for _ in range(50):
+        L = int(rng.integers(1, 33))
+        pfm = rng.integers(0, 1000, size=(L, 4)).astype(np.uint64)
+        bg = rng.integers(0, 10**10, size=4).astype(np.uint64)
+        pc = float(rng.choice([0.25, 0.1, 1.0]))
+        for rc in (False, True):
+            p = synth.reverse_complement_pfm(pfm) if rc else pfm
+            assert np.array_equal(ba.pwm(p, bg, pc, rc), orc.pfm2pwm(p, bg, pc, rc))
+
+
+def test_pvalue_threshold_matches_oracle(orc):
+    """Product -pt threshold == oracle hist + the .dat round trip + cutoff."""
+    rng = np.random.default_rng(23)
+    ms = synth.motifs(25, 4, 30, 0.4, seed=23)
+    for pfm in ms.pfms:
+        bg = rng.integers(10**5, 10**7, size=4).astype(np.uint64)
+        w = orc.pfm2pwm(pfm, bg, 0.25)
+        tot = np.float32(int(bg.sum())) + np.float32(1.0)
+        prob = np.array([(np.float32(int(x)) + np.float32(0.25)) / tot for x in bg], np.float32)
+        counts, mn, mx = orc.theoretical_hist(w, prob, 250, 10_000_000)
+        mn_w, mx_w = float(np.float32(float(f"{mn:g}"))), float(np.float32(float(f"{mx:g}")))
+        for p in (1e-3, 1e-4, 1e-5, 1e-6):
+            want = orc.score_cutoff(counts, mn_w, mx_w, p)
+            got = ba.pvalue_threshold(w, bg, p)
+            assert np.float32(got) == np.float32(want), (p, got, want)
+
+
+def test_pack_matches_numpy():
+    rng = np.random.default_rng(4)
+    for n in (0, 1, 31, 32, 33, 63, 64, 65, 1000):
+        codes = rng.integers(0, 4, size=n)
+        lower = rng.random(n) < 0.2
+        text = bytearray(synth.CODE2BASE[codes].tobytes())
+        for i in np.nonzero(lower)[0]:
+            text[i] |= 0x20
+        packed, low = ba.pack(bytes(text))
+        assert np.array_equal(packed, ba.pack_codes(codes.astype(np.uint8))[: (n + 31) // 32])
+        if lower.any():
+            bits = np.zeros(((n + 63) // 64) * 64, bool)
+            bits[:n] = lower
+            want = np.packbits(bits.reshape(-1, 8)[:, ::-1]).view(np.uint64) if n else np.zeros(0, np.uint64)
+            assert np.array_equal(low, want)
+        else:
+            assert low is None
+
+
+def test_scan_cli_errors(tmp_path):
+    d = str(copy_example(str(tmp_path)))
+    r = run_blamm(["scan", "-pt", "1e-4", "motifs.jaspar", "sequences.mf"], d, check=False)
+    assert r.returncode == 1 and "Cannot open file" in r.stderr     # no .dict yet
+    run_blamm(["dict", "sequences.mf"], d)
+    r = run_blamm(["scan", "-pt", "1e-4", "motifs.jaspar", "sequences.mf"], d, check=False)
+    assert r.returncode == 1 and "Did you run the hist module?" in r.stderr
+    r = run_blamm(["scan", "-rt", "1.5", "motifs.jaspar", "sequences.mf"], d, check=False)
+    assert r.returncode == 1 and "range [0..1]" in r.stderr
+    r = run_blamm(["scan", "-at", "3", "-pt", "1e-4", "motifs.jaspar", "sequences.mf"], d, check=False)
+    assert r.returncode == 1 and "not both" in r.stderr
+    r = run_blamm(["scan", "-x", "motifs.jaspar", "sequences.mf"], d, check=False)
+    assert r.returncode == 1 and "Usage: blamm scan" in r.stdout
+    if ba.hip_lib() and _no_gpu():
+        r = run_blamm(["scan", "-rt", "0.9", "motifs.jaspar", "sequences.mf"], d, check=False)
+        assert r.returncode == 1 and "no devices" in r.stderr
+
+
+def _no_gpu():
+    n = ctypes.c_int(0)
+    ba.hip_lib().blamm_hip_device_count(ctypes.byref(n))
+    return n.value == 0
+
+
+def test_bad_fasta_is_reported(tmp_path):
+    d = str(tmp_path)
+    with open(os.path.join(d, "x.fa"), "w") as f:
+        f.write("ACGT\n>s\nAC\n")
+    with open(os.path.join(d, "sequences.mf"), "w") as f:
+        f.write(f"g\t{d}/x.fa\n")
+    r = run_blamm(["dict", "sequences.mf"], d, check=False)
+    assert r.returncode == 1 and "fasta format" in r.stderr

@@ -1,0 +1,25 @@
+"""Plant plausible mistakes in the oracle (compile-time switches) and show
+that the pinning checks catch each one.  CPU only."""
+import pytest
+
+import oracle
+import pinning
+
+# planted mistake -> the check that must catch it
+MUTATIONS = {
+    "ORACLE_MUTATE_GT": "check_scan_equality_threshold",
+    "ORACLE_MUTATE_NO_BG_SWAP": "check_pwm_revcompl",
+    "ORACLE_MUTATE_LOWER_COUNTS": "check_scan_lowercase_zero",
+    "ORACLE_MUTATE_NO_RUN_CHECK": "check_scan_bruteforce",
+    "ORACLE_MUTATE_CUTOFF_DOUBLE": "check_cutoff_float_coordinate",
+    "ORACLE_MUTATE_HIST_ADD": "check_hist_store_not_add",
+    "ORACLE_MUTATE_RUNS_IGNORE_RECORD": "check_fasta_cases",
+}
+
+
+@pytest.mark.parametrize("define", sorted(MUTATIONS))
+def test_mutation_is_caught(define, tmp_path):
+    path = oracle.build_variant(define, str(tmp_path))
+    mutant = oracle.Oracle(path)
+    with pytest.raises(AssertionError):
+        getattr(pinning, MUTATIONS[define])(mutant)
