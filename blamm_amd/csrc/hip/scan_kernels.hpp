@@ -85,7 +85,10 @@ struct FilterArgs {
 // own LDS slice and reserves global space with one atomic per flush.  Only
 // the owning wave touches its slice; LDS operations of one wave complete in
 // program order, so no barrier is needed.
-constexpr int kStage = 512;
+constexpr int kStage = 1536;
+// A 32x32 tile of one column block can add at most this many candidates; the
+// stage is flushed before a block's emission when it could not take them.
+constexpr int kTileCands = 1024;
 
 struct Stage {
     uint2* buf;        // this wave's LDS slice
@@ -106,26 +109,28 @@ __device__ __forceinline__ void stage_flush(const FilterArgs& a, Stage& st)
     st.n = 0;
 }
 
-// Emit (window, slot) for every accumulator entry clearing its threshold.
-// Called only when the wave holds at least one such entry.  `full` (wave
-// uniform) says every window of the tile lies inside the stream.
-template <int NREG>
-__device__ __forceinline__ void emit_candidates(const FilterArgs& a, Stage& st, const float* acc, float thr,
-                                                uint32_t q_of_reg0, const uint32_t* row_of_reg,
-                                                uint32_t slot, bool full)
+// Append the candidates of one column block of a finished tile.  `bits`
+// holds, per lane, one bit per accumulator row that cleared the threshold;
+// row r of the lane is window q0 + rowq(r).  Lanes take their lowest set bit
+// each round, so the loop runs max-popcount times (almost always once).
+// `full` (wave uniform) says every window of the tile lies inside the stream.
+template <typename RowQ>
+__device__ __forceinline__ void emit_bits(const FilterArgs& a, Stage& st, uint32_t bits, RowQ rowq,
+                                          uint32_t slot, bool full)
 {
-#pragma unroll
-    for (int r = 0; r < NREG; ++r) {
-        const uint32_t q = q_of_reg0 + row_of_reg[r];
-        const bool pred = (acc[r] >= thr) && (full || q < a.nbases);
+    if (st.n > (uint32_t)(kStage - kTileCands)) stage_flush(a, st);
+    for (;;) {
+        const bool have = bits != 0;
+        if (!__ballot(have)) break;
+        const uint32_t r = (uint32_t)__builtin_ctz(bits | 0x80000000u);
+        bits &= bits - 1;
+        const uint32_t q = rowq(r);
+        const bool pred = have && (full || q < a.nbases);
         const uint64_t m = __ballot(pred);
-        if (m) {
-            if (st.n > (uint32_t)(kStage - 64)) stage_flush(a, st);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (pred) st.buf[st.n + rank] = make_uint2(q, slot);
-            st.n += (uint32_t)__popcll(m);
-        }
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (pred) st.buf[st.n + rank] = make_uint2(q, slot);
+        st.n += (uint32_t)__popcll(m);
     }
 }
 
@@ -157,50 +162,75 @@ constexpr int kGroup = 4;
 // streams window tiles through them.
 // ---------------------------------------------------------------------------
 template <int NCH, int NB, bool MASK>
-__device__ __forceinline__ void bf16_tile(const FilterArgs& a, Stage& st, const bf16x8 (&bh)[NB][NCH],
-                                          const bf16x8 (&bl)[NB][NCH], const float (&th)[NB],
-                                          const uint32_t (&rows)[16], uint32_t q0, uint64_t bits,
-                                          uint32_t mb, uint32_t blk0, uint32_t lane)
+__device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const uint4* onehot,
+                                               const bf16x8 (&bh)[NB][NCH], const bf16x8 (&bl)[NB][NCH],
+                                               uint64_t bits, uint32_t mb)
 {
-    f32x16 acc[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
 #pragma unroll
     for (int j = 0; j < NCH; ++j) {
-        const uint32_t b0 = (uint32_t)(bits >> (8 * j)) & 3u;
-        const uint32_t b1 = (uint32_t)(bits >> (8 * j + 2)) & 3u;
-        uint64_t o0 = 0x3F80ull << (b0 * 16);   // bf16 1.0 in lane b0 of 4
-        uint64_t o1 = 0x3F80ull << (b1 * 16);
-        if (MASK) {
-            if ((mb >> (4 * j)) & 1) o0 = 0;
-            if ((mb >> (4 * j + 1)) & 1) o1 = 0;
-        }
-        const u32x4 av = {(uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32)};
-        const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+        // the lane's two bases of chunk j (and their lowercase bits) index the
+        // one-hot table: one LDS read instead of building 8 bf16 lanes
+        uint32_t e = (uint32_t)(bits >> (8 * j)) & 15u;
+        if (MASK) e |= ((mb >> (4 * j)) & 3u) << 4;
+        const bf16x8 A = __builtin_bit_cast(bf16x8, onehot[e]);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bh[b][j], acc[b], 0, 0, 0);
             acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bl[b][j], acc[b], 0, 0, 0);
         }
     }
+}
+
+// Threshold test of one finished tile; D lane l holds column l%32 and rows
+// 8(r/4) + 4(l/32) + r%4.
+template <int NB>
+__device__ __forceinline__ void bf16_epilogue(const FilterArgs& a, Stage& st, const f32x16 (&acc)[NB],
+                                              const float (&th)[NB], uint32_t q0, uint32_t blk0, uint32_t lane)
+{
     const bool full = q0 + kBf16Tile <= a.nbases;
+    const uint32_t qlane = q0 + 4 * (lane >> 5);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         float mx = acc[b][0];
 #pragma unroll
         for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[b][r]);
         if (__ballot(mx >= th[b])) {
-            float v[16];
+            uint32_t bits = 0;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) v[r] = acc[b][r];
-            emit_candidates<16>(a, st, v, th[b], q0, rows, (blk0 + b) * kBf16Cols + (lane & 31), full);
+            for (int r = 0; r < 16; ++r) bits |= (acc[b][r] >= th[b]) ? (1u << r) : 0u;
+            emit_bits(a, st, bits, [qlane](uint32_t r) { return qlane + 8 * (r >> 2) + (r & 3); },
+                      (blk0 + b) * kBf16Cols + (lane & 31), full);
         }
     }
 }
 
+// One-hot bf16 fragments for two bases (and two lowercase bits): entry
+// e = b0 | b1 << 2 | m0 << 4 | m1 << 5 holds 1.0 in lane b0 of the first four
+// bf16 and in lane b1 of the last four, zeroed for a lowercase base.
+__device__ __forceinline__ uint4 onehot_entry(uint32_t e)
+{
+    const uint32_t b0 = e & 3u, b1 = (e >> 2) & 3u;
+    uint64_t o0 = 0x3F80ull << (b0 * 16);
+    uint64_t o1 = 0x3F80ull << (b1 * 16);
+    if ((e >> 4) & 1u) o0 = 0;
+    if ((e >> 5) & 1u) o1 = 0;
+    return make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
+}
+
+// The tile loop is software-pipelined over two accumulator sets: the MFMAs
+// of tile t are issued before the threshold test of tile t-1, so the
+// epilogue's VALU work and its branch overlap matrix-core work of the same
+// wave.  Packed words are read two tiles ahead.
 template <int NCH, int NB, bool MASK>
 __global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
 {
+    // 16 entries (no lowercase) span all 64 LDS banks once, so the lanes'
+    // lookups are conflict-free (equal entries broadcast).
+    __shared__ uint4 onehot[MASK ? 64 : 16];
+    if (threadIdx.x < (MASK ? 64u : 16u)) onehot[threadIdx.x] = onehot_entry(threadIdx.x);
+    __syncthreads();
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (wave >= a.nwaves) return;
@@ -228,49 +258,50 @@ __global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
         }
         th[b] = a.thr_lo[(blk0 + b) * kBf16Cols + (lane & 31)];
     }
-    const uint32_t m = lane & 31, hh = lane >> 5;
-    uint32_t rows[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) rows[r] = 8 * (r / 4) + 4 * hh + (r % 4);
 
-    const uint32_t off = m + 2 * hh;                 // first base of the lane's window, 0..33
+    const uint32_t off = (lane & 31) + 2 * (lane >> 5);   // first base of the lane's window, 0..33
     const uint32_t woff = off >> 5;
     const uint32_t sh = 2 * (off & 31), sh_inv = 63 - sh;
     const uint64_t* __restrict__ pk = a.packed + woff;
     const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + woff;
-    uint64_t w[kGroup + 1];
-    uint32_t mw[kGroup + 1];
-#pragma unroll
-    for (int u = 0; u <= kGroup; ++u) {
-        w[u] = pk[t0 + u];
-        mw[u] = MASK ? mk[t0 + u] : 0u;
+    // rolling words: tile t reads (pk[t], pk[t+1]); on loop entry
+    // (w0, w1, w2) = pk[t..t+2] and pk[t+3], pk[t+4] are fetched for the
+    // next iteration
+    uint32_t m0 = 0, m1 = 0, m2 = 0;
+    f32x16 accA[NB], accB[NB];
+    {
+        const uint64_t p0 = pk[t0];
+        const uint32_t q0m = MASK ? mk[t0] : 0u;
+        uint64_t p1 = pk[t0 + 1];
+        if (MASK) m0 = mk[t0 + 1];
+        bf16_mfma_tile<NCH, NB, MASK>(accA, onehot, bh, bl, funnel64(p0, p1, sh, sh_inv),
+                                      MASK ? funnel32(q0m, m0, off & 31) : 0u);
     }
-    for (uint32_t t = t0; t < t1; t += kGroup) {
-        uint64_t nx[kGroup];
-        uint32_t nm[kGroup];
-        if (t + kGroup < t1) {
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) {
-                nx[u] = pk[t + kGroup + 1 + u];
-                nm[u] = MASK ? mk[t + kGroup + 1 + u] : 0u;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kGroup; ++u) {
-            if (t + u < t1) {
-                const uint64_t bits = funnel64(w[u], w[u + 1], sh, sh_inv);
-                const uint32_t mb = MASK ? funnel32(mw[u], mw[u + 1], off & 31) : 0u;
-                bf16_tile<NCH, NB, MASK>(a, st, bh, bl, th, rows, (t + u) * kBf16Tile, bits, mb, blk0, lane);
-            }
-        }
-        w[0] = w[kGroup];
-        mw[0] = mw[kGroup];
-#pragma unroll
-        for (int u = 0; u < kGroup; ++u) {
-            w[u + 1] = nx[u];
-            mw[u + 1] = nm[u];
-        }
+    uint64_t w0 = pk[t0 + 1], w1 = pk[t0 + 2], w2 = pk[t0 + 3];
+    if (MASK) { m1 = mk[t0 + 2]; m2 = mk[t0 + 3]; }
+    uint32_t t = t0 + 1;
+    bool pendingA = true;           // accA holds tile t-1, not yet tested
+    for (; t + 1 < t1; t += 2) {
+        const uint64_t w3 = pk[t + 3], w4 = pk[t + 4];
+        uint32_t m3 = 0, m4 = 0;
+        if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
+        bf16_mfma_tile<NCH, NB, MASK>(accB, onehot, bh, bl, funnel64(w0, w1, sh, sh_inv),
+                                      MASK ? funnel32(m0, m1, off & 31) : 0u);
+        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
+        bf16_mfma_tile<NCH, NB, MASK>(accA, onehot, bh, bl, funnel64(w1, w2, sh, sh_inv),
+                                      MASK ? funnel32(m1, m2, off & 31) : 0u);
+        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
+        w0 = w2; w1 = w3; w2 = w4;
+        if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
+    if (t < t1) {                   // one tile left: t
+        bf16_mfma_tile<NCH, NB, MASK>(accB, onehot, bh, bl, funnel64(w0, w1, sh, sh_inv),
+                                      MASK ? funnel32(m0, m1, off & 31) : 0u);
+        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
+        pendingA = false;
+        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
+    }
+    if (pendingA) bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
     stage_flush(a, st);
 }
 
@@ -282,7 +313,7 @@ __global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
 // ---------------------------------------------------------------------------
 template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, const float (&bw)[NB][4 * NCH],
-                                             const float (&th)[NB], const uint32_t (&rows)[4], uint32_t q0,
+                                             const float (&th)[NB], uint32_t q0,
                                              uint64_t bits, uint32_t mb, uint32_t k, uint32_t blk0,
                                              uint32_t lane)
 {
@@ -301,12 +332,16 @@ __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, con
             acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bw[b][p], acc[b], 0, 0, 0);
     }
     const bool full = q0 + kFp32Tile <= a.nbases;
+    const uint32_t qlane = q0 + 4 * k;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const float mx = fmaxf(fmaxf(acc[b][0], acc[b][1]), fmaxf(acc[b][2], acc[b][3]));
         if (__ballot(mx >= th[b])) {
-            float v[4] = {acc[b][0], acc[b][1], acc[b][2], acc[b][3]};
-            emit_candidates<4>(a, st, v, th[b], q0, rows, (blk0 + b) * kFp32Cols + (lane & 15), full);
+            uint32_t bits = 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) bits |= (acc[b][r] >= th[b]) ? (1u << r) : 0u;
+            emit_bits(a, st, bits, [qlane](uint32_t r) { return qlane + r; },
+                      (blk0 + b) * kFp32Cols + (lane & 15), full);
         }
     }
 }
@@ -337,9 +372,6 @@ __global__ __launch_bounds__(kThreads) void filter_fp32_kernel(FilterArgs a)
         th[b] = a.thr_lo[(blk0 + b) * kFp32Cols + (lane & 15)];
     }
     const uint32_t i = lane & 15, k = lane >> 4;
-    uint32_t rows[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) rows[r] = 4 * k + r;
     const uint32_t s0 = 2 * i, s1 = 2 * (i + 16);    // bit offsets of the two sub-tiles
     const uint64_t* __restrict__ pk = a.packed;
     const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask;
@@ -368,8 +400,8 @@ __global__ __launch_bounds__(kThreads) void filter_fp32_kernel(FilterArgs a)
                 const uint64_t b1 = funnel64(w[u], w[u + 1], s1, 63 - s1);
                 const uint32_t m0 = MASK ? funnel32(mw[u], mw[u + 1], i) : 0u;
                 const uint32_t m1 = MASK ? funnel32(mw[u], mw[u + 1], i + 16) : 0u;
-                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, rows, q0, b0, m0, k, blk0, lane);
-                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, rows, q0 + kFp32Tile, b1, m1, k, blk0, lane);
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0, b0, m0, k, blk0, lane);
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0 + kFp32Tile, b1, m1, k, blk0, lane);
             }
         }
         w[0] = w[kGroup];
