@@ -108,7 +108,7 @@ double gamma_n(double n, double u) { return n * u / (1.0 - n * u); }
 struct blamm_hip_ctx {
     int device = 0;
     int num_cu = 256;
-    int waves_per_cu = 16;      // launch size: waves per CU per class launch
+    int filter_rounds = 1;      // rounds of resident waves per filter launch
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
@@ -140,6 +140,30 @@ struct blamm_hip_ctx {
 };
 
 namespace {
+
+// Workgroups of a filter kernel resident per CU (register and LDS limits).
+template <int NCH, bool MASK>
+int filter_wgs_per_cu_t(bool bf)
+{
+    int n = 0;
+    hipError_t e = bf ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_bf16x2_kernel<NCH, kNbBf16, MASK>,
+                                                                     kThreads, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_fp32_kernel<NCH, kNbFp32, MASK>,
+                                                                     kThreads, 0);
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+int filter_wgs_per_cu(bool bf, bool mask, int nch)
+{
+    switch (nch * 2 + (mask ? 1 : 0)) {
+#define BLAMM_OCC(N) \
+    case 2 * N: return filter_wgs_per_cu_t<N, false>(bf); \
+    case 2 * N + 1: return filter_wgs_per_cu_t<N, true>(bf);
+    BLAMM_OCC(1) BLAMM_OCC(2) BLAMM_OCC(3) BLAMM_OCC(4) BLAMM_OCC(5) BLAMM_OCC(6) BLAMM_OCC(7) BLAMM_OCC(8)
+#undef BLAMM_OCC
+    }
+    return 1;
+}
 
 template <int NCH, bool MASK>
 hipError_t launch_filter_t(bool bf, const FilterArgs& a, uint32_t grid, hipStream_t s)
@@ -216,9 +240,9 @@ int blamm_hip_create(int device, blamm_hip_ctx** out)
     blamm_hip_ctx* c = new blamm_hip_ctx();
     c->device = device;
     c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    if (const char* e = std::getenv("BLAMM_WAVES_PER_CU")) {
+    if (const char* e = std::getenv("BLAMM_FILTER_ROUNDS")) {
         const int v = std::atoi(e);
-        if (v >= 1 && v <= 1024) c->waves_per_cu = v;
+        if (v >= 1 && v <= 64) c->filter_rounds = v;
     }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
@@ -492,9 +516,13 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
             a.nbases = (uint32_t)nbases;
             a.ntiles = ntiles;
             a.ngroups = (uint32_t)(ci.nblk / nb);
-            const uint64_t target = (uint64_t)c->num_cu * (uint64_t)c->waves_per_cu;
-            uint64_t ranges = (target + a.ngroups - 1) / a.ngroups;
-            ranges = std::max<uint64_t>(1, std::min<uint64_t>(ranges, ntiles));
+            // One round of resident waves: every wave slot of the chip gets one
+            // (column group, tile range) item of equal size, so the launch has
+            // no partial last round.
+            const uint64_t slots = (uint64_t)c->num_cu * kWavesPerBlock *
+                                   (uint64_t)filter_wgs_per_cu(bf, mask, ci.nch) * (uint64_t)c->filter_rounds;
+            uint64_t ranges = std::max<uint64_t>(1, slots / a.ngroups);
+            ranges = std::min<uint64_t>(ranges, ntiles);
             a.tiles_per_wave = (uint32_t)((ntiles + ranges - 1) / ranges);
             ranges = (ntiles + a.tiles_per_wave - 1) / a.tiles_per_wave;
             a.nwaves = (uint32_t)(ranges * a.ngroups);
@@ -531,7 +559,10 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
         f.hits = d_hits;
         f.hit_cap = cap;
         const uint32_t grid = (uint32_t)c->num_cu * 8;
-        hipLaunchKernelGGL(finalize_kernel, dim3(grid), dim3(kThreads), 0, s, f);
+        if (bf)
+            hipLaunchKernelGGL(finalize_kernel<16>, dim3(grid), dim3(kThreads), 0, s, f);
+        else
+            hipLaunchKernelGGL(finalize_kernel<4>, dim3(grid), dim3(kThreads), 0, s, f);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(c->ev[2], s));

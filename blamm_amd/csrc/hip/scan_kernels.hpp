@@ -85,10 +85,7 @@ struct FilterArgs {
 // own LDS slice and reserves global space with one atomic per flush.  Only
 // the owning wave touches its slice; LDS operations of one wave complete in
 // program order, so no barrier is needed.
-constexpr int kStage = 1536;
-// A 32x32 tile of one column block can add at most this many candidates; the
-// stage is flushed before a block's emission when it could not take them.
-constexpr int kTileCands = 1024;
+constexpr int kStage = 512;
 
 struct Stage {
     uint2* buf;        // this wave's LDS slice
@@ -109,27 +106,20 @@ __device__ __forceinline__ void stage_flush(const FilterArgs& a, Stage& st)
     st.n = 0;
 }
 
-// Append the candidates of one column block of a finished tile.  `bits`
-// holds, per lane, one bit per accumulator row that cleared the threshold;
-// row r of the lane is window q0 + rowq(r).  Lanes take their lowest set bit
-// each round, so the loop runs max-popcount times (almost always once).
-// `full` (wave uniform) says every window of the tile lies inside the stream.
-template <typename RowQ>
-__device__ __forceinline__ void emit_bits(const FilterArgs& a, Stage& st, uint32_t bits, RowQ rowq,
-                                          uint32_t slot, bool full)
+// Record the lanes whose accumulator maximum clears the filter threshold.
+// A record (first window of the lane, slot) stands for all windows the lane
+// holds (16 for the bf16 tile, 4 for the fp32 sub-tile); finalize_kernel
+// resolves them exactly.  Keeping row resolution out of the filter keeps its
+// rare path to a handful of instructions.
+__device__ __forceinline__ void emit_lane(const FilterArgs& a, Stage& st, bool pred, uint32_t qlane,
+                                          uint32_t slot)
 {
-    if (st.n > (uint32_t)(kStage - kTileCands)) stage_flush(a, st);
-    for (;;) {
-        const bool have = bits != 0;
-        if (!__ballot(have)) break;
-        const uint32_t r = (uint32_t)__builtin_ctz(bits | 0x80000000u);
-        bits &= bits - 1;
-        const uint32_t q = rowq(r);
-        const bool pred = have && (full || q < a.nbases);
-        const uint64_t m = __ballot(pred);
+    const uint64_t m = __ballot(pred);
+    if (m) {
+        if (st.n > (uint32_t)(kStage - 64)) stage_flush(a, st);
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        if (pred) st.buf[st.n + rank] = make_uint2(q, slot);
+        if (pred) st.buf[st.n + rank] = make_uint2(qlane, slot);
         st.n += (uint32_t)__popcll(m);
     }
 }
@@ -152,6 +142,17 @@ __device__ __forceinline__ uint32_t funnel32(uint32_t lo, uint32_t hi, uint32_t 
 // words and prefetches the next group's words one group ahead.
 constexpr int kGroup = 4;
 
+// Tile loop of the bf16 filter: 1 = two accumulator sets (software pipeline),
+// 0 = one set (fewer registers, more resident waves).
+#ifndef BLAMM_PIPELINE
+#define BLAMM_PIPELINE 1
+#endif
+
+// One-hot operand source of the bf16 filter: 1 = LDS table, 0 = VALU.
+#ifndef BLAMM_ONEHOT_LDS
+#define BLAMM_ONEHOT_LDS 0
+#endif
+
 // ---------------------------------------------------------------------------
 // Split-BF16 filter: W = hi + lo (bf16 each); one-hot operand is exact in bf16.
 // MFMA v_mfma_f32_32x32x16_bf16: A = one-hot (32 windows x 16 = 4 positions x
@@ -170,11 +171,28 @@ __device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const uint4* o
     for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
 #pragma unroll
     for (int j = 0; j < NCH; ++j) {
+#if BLAMM_ONEHOT_LDS
         // the lane's two bases of chunk j (and their lowercase bits) index the
-        // one-hot table: one LDS read instead of building 8 bf16 lanes
+        // one-hot table in LDS
         uint32_t e = (uint32_t)(bits >> (8 * j)) & 15u;
         if (MASK) e |= ((mb >> (4 * j)) & 3u) << 4;
         const bf16x8 A = __builtin_bit_cast(bf16x8, onehot[e]);
+#else
+        // built in registers: bf16 1.0 (0x3F80) in lane b of four per base;
+        // no memory latency, and the VALU work overlaps the previous chunk's
+        // MFMAs
+        (void)onehot;
+        const uint32_t b0 = (uint32_t)(bits >> (8 * j)) & 3u;
+        const uint32_t b1 = (uint32_t)(bits >> (8 * j + 2)) & 3u;
+        uint64_t o0 = 0x3F80ull << (b0 * 16);
+        uint64_t o1 = 0x3F80ull << (b1 * 16);
+        if (MASK) {
+            if ((mb >> (4 * j)) & 1) o0 = 0;
+            if ((mb >> (4 * j + 1)) & 1) o1 = 0;
+        }
+        const u32x4 av = {(uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32)};
+        const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#endif
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bh[b][j], acc[b], 0, 0, 0);
@@ -184,25 +202,18 @@ __device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const uint4* o
 }
 
 // Threshold test of one finished tile; D lane l holds column l%32 and rows
-// 8(r/4) + 4(l/32) + r%4.
+// 8(r/4) + 4(l/32) + r%4, i.e. windows qlane + {0..3, 8..11, 16..19, 24..27}.
 template <int NB>
 __device__ __forceinline__ void bf16_epilogue(const FilterArgs& a, Stage& st, const f32x16 (&acc)[NB],
                                               const float (&th)[NB], uint32_t q0, uint32_t blk0, uint32_t lane)
 {
-    const bool full = q0 + kBf16Tile <= a.nbases;
     const uint32_t qlane = q0 + 4 * (lane >> 5);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         float mx = acc[b][0];
 #pragma unroll
         for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[b][r]);
-        if (__ballot(mx >= th[b])) {
-            uint32_t bits = 0;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) bits |= (acc[b][r] >= th[b]) ? (1u << r) : 0u;
-            emit_bits(a, st, bits, [qlane](uint32_t r) { return qlane + 8 * (r >> 2) + (r & 3); },
-                      (blk0 + b) * kBf16Cols + (lane & 31), full);
-        }
+        emit_lane(a, st, mx >= th[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
     }
 }
 
@@ -232,7 +243,10 @@ __global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
     if (threadIdx.x < (MASK ? 64u : 16u)) onehot[threadIdx.x] = onehot_entry(threadIdx.x);
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    // readfirstlane: the wave index is uniform, so everything derived from it
+    // (range, column block, loop bounds) lives in scalar registers and the
+    // tile loop is a scalar loop
+    const uint32_t wave = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wave >= a.nwaves) return;
     __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
     Stage st{stage_lds[threadIdx.x >> 6], 0u};
@@ -268,6 +282,26 @@ __global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
     // (w0, w1, w2) = pk[t..t+2] and pk[t+3], pk[t+4] are fetched for the
     // next iteration
     uint32_t m0 = 0, m1 = 0, m2 = 0;
+#if !BLAMM_PIPELINE
+    {
+        // one accumulator set: fewer registers, more resident waves
+        f32x16 acc[NB];
+        uint64_t c0 = pk[t0], c1 = pk[t0 + 1];
+        uint32_t cm0 = MASK ? mk[t0] : 0u, cm1 = MASK ? mk[t0 + 1] : 0u;
+        for (uint32_t t = t0; t < t1; ++t) {
+            const uint64_t c2 = pk[t + 2];
+            const uint32_t cm2 = MASK ? mk[t + 2] : 0u;
+            bf16_mfma_tile<NCH, NB, MASK>(acc, onehot, bh, bl, funnel64(c0, c1, sh, sh_inv),
+                                          MASK ? funnel32(cm0, cm1, off & 31) : 0u);
+            bf16_epilogue<NB>(a, st, acc, th, t * kBf16Tile, blk0, lane);
+            c0 = c1; c1 = c2;
+            if (MASK) { cm0 = cm1; cm1 = cm2; }
+        }
+        stage_flush(a, st);
+        (void)m0; (void)m1; (void)m2;
+        return;
+    }
+#endif
     f32x16 accA[NB], accB[NB];
     {
         const uint64_t p0 = pk[t0];
@@ -331,18 +365,11 @@ __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, con
         for (int b = 0; b < NB; ++b)
             acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bw[b][p], acc[b], 0, 0, 0);
     }
-    const bool full = q0 + kFp32Tile <= a.nbases;
-    const uint32_t qlane = q0 + 4 * k;
+    const uint32_t qlane = q0 + 4 * k;     // windows qlane + 0..3
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const float mx = fmaxf(fmaxf(acc[b][0], acc[b][1]), fmaxf(acc[b][2], acc[b][3]));
-        if (__ballot(mx >= th[b])) {
-            uint32_t bits = 0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) bits |= (acc[b][r] >= th[b]) ? (1u << r) : 0u;
-            emit_bits(a, st, bits, [qlane](uint32_t r) { return qlane + r; },
-                      (blk0 + b) * kFp32Cols + (lane & 15), full);
-        }
+        emit_lane(a, st, mx >= th[b], qlane, (blk0 + b) * kFp32Cols + (lane & 15));
     }
 }
 
@@ -350,7 +377,10 @@ template <int NCH, int NB, bool MASK>
 __global__ __launch_bounds__(kThreads) void filter_fp32_kernel(FilterArgs a)
 {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    // readfirstlane: the wave index is uniform, so everything derived from it
+    // (range, column block, loop bounds) lives in scalar registers and the
+    // tile loop is a scalar loop
+    const uint32_t wave = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wave >= a.nwaves) return;
     __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
     Stage st{stage_lds[threadIdx.x >> 6], 0u};
@@ -443,9 +473,20 @@ struct DevHit {
 
 constexpr int kHitStage = 256;
 
-// Exact decision for each candidate: run-boundary rule (pwmscan.cpp:122-126),
-// then the left-to-right FP32 sum; windows whose FP32 sum lies within its
-// proven band of the threshold are rescored in FP64 (the only FP64 step).
+// Windows a filter record stands for: record window q covers q + off(r).
+template <int NR>
+__device__ __forceinline__ uint32_t record_offset(int r)
+{
+    return NR == 16 ? (uint32_t)(8 * (r >> 2) + (r & 3)) : (uint32_t)r;
+}
+
+// Exact decision for every window of every filter record: the run-boundary
+// rule (a window must end at or before the next run start, pwmscan.cpp:122-126),
+// then the left-to-right FP32 sum; a window whose FP32 sum lies within its
+// proven band of the threshold is rescored in FP64 (the only FP64 step).
+// One thread per record; its NR windows share the column's weights and a
+// 128-bit window of packed bases that is shifted one position per step.
+template <int NR>
 __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
 {
     const uint64_t ncand_raw = a.counters[kCntCand];
@@ -467,62 +508,107 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             if ((uint64_t)at0 + i < a.hit_cap) hits[at0 + i] = sbuf[i];
         sn = 0;
     };
+    constexpr uint32_t span = NR == 16 ? 28 : 4;    // windows covered by a record
     for (uint64_t base = wave * 64; base < ncand; base += nw * 64) {
         const uint64_t idx = base + lane;
-        bool hit = false;
-        float score = 0.f;
-        uint32_t q = 0, slot = 0;
-        if (idx < ncand) {
+        const bool have = idx < ncand;
+        uint32_t q0 = 0, slot = 0;
+        if (have) {
             const uint2 cd = a.cand[idx];
-            q = cd.x;
+            q0 = cd.x;
             slot = cd.y;
-            const int L = a.slot_len[slot];
-            uint32_t lo = 0, hi = a.nruns;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (a.run_starts[mid] <= q) lo = mid + 1; else hi = mid;
+        }
+        const int L = have ? a.slot_len[slot] : 0;
+        const float* w = a.slot_w + (size_t)slot * kMaxLen * 4;
+        // packed bases q0 .. q0+63 and their lowercase bits
+        uint64_t lo = 0, hi = 0, mlo = 0;
+        uint32_t nb1 = 0;
+        if (have) {
+            lo = load_bits64(a.packed, 2ull * q0);
+            hi = load_bits64(a.packed, 2ull * q0 + 64);
+            if (a.lcmask) mlo = load_bits64(a.lcmask, q0);
+            uint32_t l = 0, h = a.nruns;
+            while (l < h) {
+                const uint32_t mid = (l + h) >> 1;
+                if (a.run_starts[mid] <= q0) l = mid + 1; else h = mid;
             }
-            const uint64_t nxt = lo < a.nruns ? a.run_starts[lo] : (uint64_t)a.nbases;
-            if ((uint64_t)q + (uint64_t)L <= nxt) {
-                const uint64_t bits = load_bits64(a.packed, 2ull * q);
-                const uint64_t mb = a.lcmask ? load_bits64(a.lcmask, q) : 0ull;
-                const float* w = a.slot_w + (size_t)slot * kMaxLen * 4;
-                float s = 0.f;
-                for (int p = 0; p < L; ++p) {
-                    if ((mb >> p) & 1) continue;
-                    s += w[4 * p + ((bits >> (2 * p)) & 3)];
-                }
-                if (s >= a.slot_seq_hi[slot]) {
-                    hit = true;
-                    score = s;
-                } else if (s >= a.slot_seq_lo[slot]) {
-                    double d = 0.0;
-                    for (int p = 0; p < L; ++p) {
-                        if ((mb >> p) & 1) continue;
-                        d += (double)w[4 * p + ((bits >> (2 * p)) & 3)];
+            nb1 = l < a.nruns ? (uint32_t)a.run_starts[l] : a.nbases;
+        }
+        float s[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) s[r] = 0.f;
+        uint64_t wl = lo, wh = hi, ml = mlo;
+        for (int p = 0; p < L; ++p) {
+            const float4 wp = *(const float4*)(w + 4 * p);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const uint32_t o = record_offset<NR>(r);
+                const uint32_t b = (uint32_t)(wl >> (2 * o)) & 3u;
+                const float x = b == 0 ? wp.x : b == 1 ? wp.y : b == 2 ? wp.z : wp.w;
+                if (!((ml >> o) & 1)) s[r] += x;
+            }
+            wl = (wl >> 2) | (wh << 62);
+            wh >>= 2;
+            ml >>= 1;
+        }
+        const float thr = have ? a.slot_thr[slot] : 0.f;
+        const float shi = have ? a.slot_seq_hi[slot] : 0.f;
+        const float slo = have ? a.slot_seq_lo[slot] : 0.f;
+        const uint32_t col = have ? a.slot_col[slot] : 0u;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const uint32_t q = q0 + record_offset<NR>(r);
+            bool hit = false;
+            float score = s[r];
+            if (have && q < a.nbases && s[r] >= slo) {
+                // run-boundary rule: next run start after q
+                uint32_t nb = nb1;
+                if (q >= nb1) {
+                    uint32_t l = 0, h = a.nruns;
+                    while (l < h) {
+                        const uint32_t mid = (l + h) >> 1;
+                        if (a.run_starts[mid] <= q) l = mid + 1; else h = mid;
                     }
-                    atomicAdd(&a.counters_rw[kCntBand], 1ull);
-                    if (d >= (double)a.slot_thr[slot]) {
+                    nb = l < a.nruns ? (uint32_t)a.run_starts[l] : a.nbases;
+                }
+                if ((uint64_t)q + (uint64_t)L <= (uint64_t)nb) {
+                    if (s[r] >= shi) {
                         hit = true;
-                        score = (float)d;
+                    } else {
+                        // within the FP32 band: FP64 left-to-right sum
+                        const uint32_t o = record_offset<NR>(r);
+                        const uint64_t bits = load_bits64(a.packed, 2ull * q);
+                        const uint64_t mb = a.lcmask ? load_bits64(a.lcmask, q) : 0ull;
+                        double d = 0.0;
+                        for (int p = 0; p < L; ++p) {
+                            if ((mb >> p) & 1) continue;
+                            d += (double)w[4 * p + ((bits >> (2 * p)) & 3)];
+                        }
+                        (void)o;
+                        atomicAdd(&a.counters_rw[kCntBand], 1ull);
+                        if (d >= (double)thr) {
+                            hit = true;
+                            score = (float)d;
+                        }
                     }
                 }
             }
-        }
-        const uint64_t m = __ballot(hit);
-        if (m) {
-            if (sn > (uint32_t)(kHitStage - 64)) flush();
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (hit) {
-                DevHit h;
-                h.pos = q;
-                h.col = a.slot_col[slot];
-                h.score = score;
-                sbuf[sn + rank] = h;
+            const uint64_t m = __ballot(hit);
+            if (m) {
+                if (sn > (uint32_t)(kHitStage - 64)) flush();
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (hit) {
+                    DevHit hh;
+                    hh.pos = q;
+                    hh.col = col;
+                    hh.score = score;
+                    sbuf[sn + rank] = hh;
+                }
+                sn += (uint32_t)__popcll(m);
             }
-            sn += (uint32_t)__popcll(m);
         }
+        (void)span;
     }
     flush();
 }
