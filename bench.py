@@ -34,6 +34,19 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per filter launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (None when absent)."""
+    path = os.path.join(REPO, "profiles", "filter_persistent_config1_traffic.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def make_workload(seed: int, nbases: int, pvalue: float):
     import blamm_amd as ba
     from blamm_amd import synth
@@ -150,7 +163,19 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (for scaling FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the scan shards by stream range with no collective; see the design notes)", file=sys.stderr)
@@ -172,6 +197,7 @@ def main():
         b = np.sort(fp_hits["pos"].astype(np.uint64) * np.uint64(1 << 16) + fp_hits["col"])
         fp["identical_hit_set"] = bool(a.size == b.size and np.array_equal(a, b))
     value = nbp * ncols / (bf["ms_per_step"] / 1e3) / 1e9
+    traffic = measured_traffic()
     achieved = flops / (bf["filter_ms"] / 1e3) / 1e12
     alg_bytes = 0.25 * nbp + 16.0 * bf["hits"]
     line = {
@@ -191,8 +217,8 @@ def main():
                                f"{args.bases // 1_000_000} Mbp GC 0.41 CpG 0.2x 0.5% N-runs, p={args.pvalue:g}",
                    "bases": nbp, "columns": ncols, "pvalue": args.pvalue, "precision": "split-bf16"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
-                     "kernel": "filter_bf16x2_kernel (all chunk classes of one step)",
+                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
+                     "kernel": "filter_persistent_kernel<true,2,false> (one launch per step)",
                      "algorithmic_flops": flops},
         "hbm": {"algorithmic_bytes": alg_bytes, "achieved_GBs": alg_bytes / (bf["kernel_ms"] / 1e3) / 1e9,
                 "peak_GBs": PEAK_HBM_GBS},

@@ -108,7 +108,8 @@ double gamma_n(double n, double u) { return n * u / (1.0 - n * u); }
 struct blamm_hip_ctx {
     int device = 0;
     int num_cu = 256;
-    int filter_rounds = 1;      // rounds of resident waves per filter launch
+    int filter_rounds = 1;      // resident-wave multiples of the persistent grid
+    uint32_t chunk_tiles = 512; // tiles per work-queue item
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
@@ -141,59 +142,38 @@ struct blamm_hip_ctx {
 
 namespace {
 
-// Workgroups of a filter kernel resident per CU (register and LDS limits).
-template <int NCH, bool MASK>
-int filter_wgs_per_cu_t(bool bf)
+template <bool BF, bool MASK>
+const void* persistent_kernel()
+{
+    return BF ? (const void*)filter_persistent_kernel<true, kNbBf16, MASK>
+              : (const void*)filter_persistent_kernel<false, kNbFp32, MASK>;
+}
+
+// Workgroups of the persistent filter resident per CU (register/LDS limits).
+int persistent_wgs_per_cu(bool bf, bool mask)
 {
     int n = 0;
-    hipError_t e = bf ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_bf16x2_kernel<NCH, kNbBf16, MASK>,
-                                                                     kThreads, 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_fp32_kernel<NCH, kNbFp32, MASK>,
-                                                                     kThreads, 0);
+    hipError_t e;
+    if (bf)
+        e = mask ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_persistent_kernel<true, kNbBf16, true>, kThreads, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_persistent_kernel<true, kNbBf16, false>, kThreads, 0);
+    else
+        e = mask ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_persistent_kernel<false, kNbFp32, true>, kThreads, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_persistent_kernel<false, kNbFp32, false>, kThreads, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-int filter_wgs_per_cu(bool bf, bool mask, int nch)
+hipError_t launch_persistent(bool bf, bool mask, const FilterArgs& a, const WorkQueue& q, uint32_t grid,
+                             hipStream_t s)
 {
-    switch (nch * 2 + (mask ? 1 : 0)) {
-#define BLAMM_OCC(N) \
-    case 2 * N: return filter_wgs_per_cu_t<N, false>(bf); \
-    case 2 * N + 1: return filter_wgs_per_cu_t<N, true>(bf);
-    BLAMM_OCC(1) BLAMM_OCC(2) BLAMM_OCC(3) BLAMM_OCC(4) BLAMM_OCC(5) BLAMM_OCC(6) BLAMM_OCC(7) BLAMM_OCC(8)
-#undef BLAMM_OCC
+    if (bf) {
+        if (mask) hipLaunchKernelGGL((filter_persistent_kernel<true, kNbBf16, true>), dim3(grid), dim3(kThreads), 0, s, a, q);
+        else hipLaunchKernelGGL((filter_persistent_kernel<true, kNbBf16, false>), dim3(grid), dim3(kThreads), 0, s, a, q);
+    } else {
+        if (mask) hipLaunchKernelGGL((filter_persistent_kernel<false, kNbFp32, true>), dim3(grid), dim3(kThreads), 0, s, a, q);
+        else hipLaunchKernelGGL((filter_persistent_kernel<false, kNbFp32, false>), dim3(grid), dim3(kThreads), 0, s, a, q);
     }
-    return 1;
-}
-
-template <int NCH, bool MASK>
-hipError_t launch_filter_t(bool bf, const FilterArgs& a, uint32_t grid, hipStream_t s)
-{
-    if (bf)
-        hipLaunchKernelGGL((filter_bf16x2_kernel<NCH, kNbBf16, MASK>), dim3(grid), dim3(kThreads), 0, s, a);
-    else
-        hipLaunchKernelGGL((filter_fp32_kernel<NCH, kNbFp32, MASK>), dim3(grid), dim3(kThreads), 0, s, a);
     return hipGetLastError();
-}
-
-template <bool MASK>
-hipError_t launch_filter_nch(bool bf, int nch, const FilterArgs& a, uint32_t grid, hipStream_t s)
-{
-    switch (nch) {
-    case 1: return launch_filter_t<1, MASK>(bf, a, grid, s);
-    case 2: return launch_filter_t<2, MASK>(bf, a, grid, s);
-    case 3: return launch_filter_t<3, MASK>(bf, a, grid, s);
-    case 4: return launch_filter_t<4, MASK>(bf, a, grid, s);
-    case 5: return launch_filter_t<5, MASK>(bf, a, grid, s);
-    case 6: return launch_filter_t<6, MASK>(bf, a, grid, s);
-    case 7: return launch_filter_t<7, MASK>(bf, a, grid, s);
-    case 8: return launch_filter_t<8, MASK>(bf, a, grid, s);
-    }
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_filter(bool bf, bool mask, int nch, const FilterArgs& a, uint32_t grid, hipStream_t s)
-{
-    return mask ? launch_filter_nch<true>(bf, nch, a, grid, s) : launch_filter_nch<false>(bf, nch, a, grid, s);
 }
 
 __global__ void store_count_kernel(const unsigned long long* counters, uint64_t* out)
@@ -240,6 +220,10 @@ int blamm_hip_create(int device, blamm_hip_ctx** out)
     blamm_hip_ctx* c = new blamm_hip_ctx();
     c->device = device;
     c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (const char* e = std::getenv("BLAMM_CHUNK_TILES")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= (1 << 20)) c->chunk_tiles = (uint32_t)v;
+    }
     if (const char* e = std::getenv("BLAMM_FILTER_ROUNDS")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 64) c->filter_rounds = v;
@@ -509,34 +493,43 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     if (nbases > 0) {
         const uint32_t ntiles = (uint32_t)((nbases + kBf16Tile - 1) / kBf16Tile);   // 32-window tiles
         const int nb = bf ? kNbBf16 : kNbFp32;
-        for (const ClassInfo& ci : c->classes) {
-            FilterArgs a;
-            a.packed = d_packed;
-            a.lcmask = d_lcmask;
-            a.nbases = (uint32_t)nbases;
-            a.ntiles = ntiles;
-            a.ngroups = (uint32_t)(ci.nblk / nb);
-            // One round of resident waves: every wave slot of the chip gets one
-            // (column group, tile range) item of equal size, so the launch has
-            // no partial last round.
-            const uint64_t slots = (uint64_t)c->num_cu * kWavesPerBlock *
-                                   (uint64_t)filter_wgs_per_cu(bf, mask, ci.nch) * (uint64_t)c->filter_rounds;
-            uint64_t ranges = std::max<uint64_t>(1, slots / a.ngroups);
-            ranges = std::min<uint64_t>(ranges, ntiles);
-            a.tiles_per_wave = (uint32_t)((ntiles + ranges - 1) / ranges);
-            ranges = (ntiles + a.tiles_per_wave - 1) / a.tiles_per_wave;
-            a.nwaves = (uint32_t)(ranges * a.ngroups);
-            a.blk_begin = (uint32_t)ci.blk_begin;
-            a.frag_a = c->frag_a.p;
-            a.frag_b = c->frag_b.p;
-            a.thr_lo = c->thr_lo.p;
-            a.cand = c->cand.p;
-            a.counters = c->counters.p;
-            a.cand_cap = c->cand.n;
-            const uint32_t grid = (a.nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
-            hipError_t e = launch_filter(bf, mask, ci.nch, a, grid, s);
-            if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("filter launch: ") + hipGetErrorString(e));
+        FilterArgs a;
+        a.packed = d_packed;
+        a.lcmask = d_lcmask;
+        a.nbases = (uint32_t)nbases;
+        a.ntiles = ntiles;
+        a.tiles_per_wave = 0;
+        a.ngroups = 0;
+        a.nwaves = 0;
+        a.blk_begin = 0;
+        a.frag_a = c->frag_a.p;
+        a.frag_b = c->frag_b.p;
+        a.thr_lo = c->thr_lo.p;
+        a.cand = c->cand.p;
+        a.counters = c->counters.p;
+        a.cand_cap = c->cand.n;
+        // work queue: column groups numbered longest motifs first
+        WorkQueue q{};
+        q.chunk_tiles = std::max<uint32_t>(1, std::min<uint32_t>(c->chunk_tiles, ntiles));
+        q.nchunks = (ntiles + q.chunk_tiles - 1) / q.chunk_tiles;
+        std::vector<ClassInfo> cls = c->classes;
+        std::sort(cls.begin(), cls.end(), [](const ClassInfo& x, const ClassInfo& y) { return x.nch > y.nch; });
+        if (cls.size() > (size_t)kMaxClasses) return fail(BLAMM_ERR_ARG, "scan: too many length classes");
+        q.nclasses = (uint32_t)cls.size();
+        uint32_t groups = 0;
+        for (size_t i = 0; i < cls.size(); ++i) {
+            q.group_base[i] = groups;
+            q.nch[i] = (uint32_t)cls[i].nch;
+            q.blk_begin[i] = (uint32_t)cls[i].blk_begin;
+            groups += (uint32_t)(cls[i].nblk / nb);
         }
+        q.group_base[cls.size()] = groups;
+        q.ngroups = groups;
+        if ((uint64_t)((q.nchunks + kXcds - 1) / kXcds) * groups >= (1ull << 32))
+            return fail(BLAMM_ERR_ARG, "scan: work queue too long");
+        const uint32_t grid = (uint32_t)c->num_cu * (uint32_t)persistent_wgs_per_cu(bf, mask) * (uint32_t)c->filter_rounds;
+        hipError_t e = launch_persistent(bf, mask, a, q, grid, s);
+        if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("filter launch: ") + hipGetErrorString(e));
     }
     HIP_TRY(hipEventRecord(c->ev[1], s));
     {

@@ -35,7 +35,7 @@ constexpr int kFp32Cols = 16;        // columns per fp32 column block
 constexpr int kFp32Tile = 16;        // windows per fp32 tile
 
 // Counter slots in device memory.
-enum { kCntCand = 0, kCntHits = 1, kCntBand = 2, kCntSlots = 4 };
+enum { kCntCand = 0, kCntHits = 1, kCntBand = 2, kCntQueue = 4, kCntSlots = 12 };
 
 // 64 consecutive bits of a little-endian bit stream starting at bit `bit`.
 __device__ __forceinline__ uint64_t load_bits64(const uint64_t* __restrict__ words, uint64_t bit)
@@ -230,33 +230,15 @@ __device__ __forceinline__ uint4 onehot_entry(uint32_t e)
     return make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
 }
 
-// The tile loop is software-pipelined over two accumulator sets: the MFMAs
-// of tile t are issued before the threshold test of tile t-1, so the
-// epilogue's VALU work and its branch overlap matrix-core work of the same
-// wave.  Packed words are read two tiles ahead.
+// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1.  The tile loop is
+// software-pipelined over two accumulator sets: the MFMAs of tile t are
+// issued before the threshold test of tile t-1, so the epilogue's VALU work
+// and its branch overlap matrix-core work of the same wave.  Packed words are
+// read two tiles ahead.
 template <int NCH, int NB, bool MASK>
-__global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
+__device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const uint4* onehot, uint32_t blk0,
+                                           uint32_t t0, uint32_t t1, uint32_t lane)
 {
-    // 16 entries (no lowercase) span all 64 LDS banks once, so the lanes'
-    // lookups are conflict-free (equal entries broadcast).
-    __shared__ uint4 onehot[MASK ? 64 : 16];
-    if (threadIdx.x < (MASK ? 64u : 16u)) onehot[threadIdx.x] = onehot_entry(threadIdx.x);
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & 63;
-    // readfirstlane: the wave index is uniform, so everything derived from it
-    // (range, column block, loop bounds) lives in scalar registers and the
-    // tile loop is a scalar loop
-    const uint32_t wave = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (wave >= a.nwaves) return;
-    __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
-    Stage st{stage_lds[threadIdx.x >> 6], 0u};
-    const uint32_t g = wave % a.ngroups;
-    const uint32_t rng = wave / a.ngroups;
-    const uint32_t t0 = rng * a.tiles_per_wave;
-    uint32_t t1 = t0 + a.tiles_per_wave;
-    if (t1 > a.ntiles) t1 = a.ntiles;
-    const uint32_t blk0 = a.blk_begin + g * NB;
-
     const uint4* __restrict__ fh = (const uint4*)a.frag_a;
     const uint4* __restrict__ fl = (const uint4*)a.frag_b;
     bf16x8 bh[NB][NCH], bl[NB][NCH];
@@ -297,7 +279,6 @@ __global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
             c0 = c1; c1 = c2;
             if (MASK) { cm0 = cm1; cm1 = cm2; }
         }
-        stage_flush(a, st);
         (void)m0; (void)m1; (void)m2;
         return;
     }
@@ -336,7 +317,6 @@ __global__ __launch_bounds__(kThreads) void filter_bf16x2_kernel(FilterArgs a)
         bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
     }
     if (pendingA) bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
-    stage_flush(a, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -373,24 +353,11 @@ __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, con
     }
 }
 
+// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1 (FP32 path).
 template <int NCH, int NB, bool MASK>
-__global__ __launch_bounds__(kThreads) void filter_fp32_kernel(FilterArgs a)
+__device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint32_t blk0, uint32_t t0, uint32_t t1,
+                                           uint32_t lane)
 {
-    const uint32_t lane = threadIdx.x & 63;
-    // readfirstlane: the wave index is uniform, so everything derived from it
-    // (range, column block, loop bounds) lives in scalar registers and the
-    // tile loop is a scalar loop
-    const uint32_t wave = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (wave >= a.nwaves) return;
-    __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
-    Stage st{stage_lds[threadIdx.x >> 6], 0u};
-    const uint32_t g = wave % a.ngroups;
-    const uint32_t rng = wave / a.ngroups;
-    const uint32_t t0 = rng * a.tiles_per_wave;
-    uint32_t t1 = t0 + a.tiles_per_wave;
-    if (t1 > a.ntiles) t1 = a.ntiles;
-    const uint32_t blk0 = a.blk_begin + g * NB;
-
     const float* __restrict__ fw = (const float*)a.frag_a;
     constexpr int NP = 4 * NCH;
     float bw[NB][NP];
@@ -440,6 +407,68 @@ __global__ __launch_bounds__(kThreads) void filter_fp32_kernel(FilterArgs a)
         for (int u = 0; u < kGroup; ++u) {
             w[u + 1] = nx[u];
             mw[u + 1] = nm[u];
+        }
+    }
+}
+
+// Work queue of the persistent filter.  The tile chunks are dealt to kXcds
+// sub-queues (chunk k to sub-queue k % kXcds); a workgroup starts on the
+// sub-queue of its XCD (workgroups are dispatched round-robin, so block b
+// runs on XCD b % kXcds) and steals from the others when it drains.  Inside a
+// sub-queue items run chunk-major: all column groups of a chunk follow each
+// other, so the chunk's packed words are fetched into that XCD's L2 once and
+// reused by every group.  Groups are numbered longest motifs first.
+constexpr int kMaxClasses = 8;
+constexpr int kXcds = 8;
+struct WorkQueue {
+    uint32_t nclasses;
+    uint32_t nchunks;
+    uint32_t chunk_tiles;
+    uint32_t ngroups;                         // column groups of all classes
+    uint32_t group_base[kMaxClasses + 1];     // first group of each class
+    uint32_t nch[kMaxClasses];
+    uint32_t blk_begin[kMaxClasses];
+};
+
+// One launch for the whole library: every resident wave pulls items until
+// all sub-queues are empty (one atomic per item).
+template <bool BF, int NB, bool MASK>
+__global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs a, WorkQueue q)
+{
+    __shared__ uint4 onehot[MASK ? 64 : 16];
+    if (BF && threadIdx.x < (MASK ? 64u : 16u)) onehot[threadIdx.x] = onehot_entry(threadIdx.x);
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63;
+    __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
+    Stage st{stage_lds[threadIdx.x >> 6], 0u};
+    const uint32_t home = blockIdx.x % kXcds;
+    for (uint32_t hop = 0; hop < (uint32_t)kXcds; ++hop) {
+        const uint32_t x = (home + hop) % kXcds;
+        if (x >= q.nchunks) continue;
+        const uint32_t items = ((q.nchunks - x + kXcds - 1) / kXcds) * q.ngroups;
+        for (;;) {
+            uint32_t j = 0;
+            if (lane == 0) j = atomicAdd((unsigned int*)&a.counters[kCntQueue + x], 1u);
+            j = __builtin_amdgcn_readfirstlane(__shfl(j, 0));
+            if (j >= items) break;
+            const uint32_t k = (j / q.ngroups) * kXcds + x;
+            const uint32_t gi = j % q.ngroups;
+            uint32_t c = 0;
+            while (c + 1 < q.nclasses && gi >= q.group_base[c + 1]) ++c;
+            const uint32_t blk0 = q.blk_begin[c] + (gi - q.group_base[c]) * NB;
+            const uint32_t t0 = k * q.chunk_tiles;
+            const uint32_t t1 = min(a.ntiles, t0 + q.chunk_tiles);
+            switch (q.nch[c]) {
+#define BLAMM_CASE(N)                                                              \
+            case N:                                                                \
+                if constexpr (BF) bf16_range<N, NB, MASK>(a, st, onehot, blk0, t0, t1, lane); \
+                else fp32_range<N, NB, MASK>(a, st, blk0, t0, t1, lane);           \
+                break;
+            BLAMM_CASE(1) BLAMM_CASE(2) BLAMM_CASE(3) BLAMM_CASE(4)
+            BLAMM_CASE(5) BLAMM_CASE(6) BLAMM_CASE(7) BLAMM_CASE(8)
+#undef BLAMM_CASE
+            default: break;
+            }
         }
     }
     stage_flush(a, st);
@@ -617,5 +646,6 @@ __global__ void zero_counters_kernel(unsigned long long* c)
 {
     if (threadIdx.x < kCntSlots) c[threadIdx.x] = 0ull;
 }
+static_assert(kCntQueue + kXcds <= kCntSlots, "queue counters");
 
 }  // namespace blamm
