@@ -50,6 +50,38 @@ class ScanStats(ctypes.Structure):
                 ("filter_ms", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    """struct blamm_tuning: window tile, motif tile, K-padding."""
+    _fields_ = [("chunk_tiles", ctypes.c_uint32), ("blocks_per_wave", ctypes.c_uint32),
+                ("kpad_positions", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {"chunk_tiles": self.chunk_tiles, "blocks_per_wave": self.blocks_per_wave,
+                "kpad_positions": self.kpad_positions}
+
+
+class Request(ctypes.Structure):
+    """struct blamm_request."""
+    _fields_ = [("packed", ctypes.c_void_p), ("lcmask", ctypes.c_void_p), ("nbases", ctypes.c_uint64),
+                ("run_starts", ctypes.c_void_p), ("nruns", ctypes.c_uint64), ("thresholds", ctypes.c_void_p),
+                ("expected_hits", ctypes.c_uint64)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("slots", ctypes.c_uint32),
+                ("max_batch_requests", ctypes.c_uint32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("batch_requests", ctypes.c_uint32), ("nhits", ctypes.c_uint64),
+                ("t_submit_ns", ctypes.c_uint64), ("t_start_ns", ctypes.c_uint64), ("t_done_ns", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("hits", ctypes.c_uint64), ("device_ms", ctypes.c_double)]
+
+
 _hip = None
 _host = None
 
@@ -99,6 +131,27 @@ def hip_lib() -> ctypes.CDLL:
                                               vp, ctypes.c_uint64, vp, ctypes.c_uint64, vp, vp]
         lib.blamm_hip_scan_finish.argtypes = [vp, _u64p]
         lib.blamm_hip_last_stats.argtypes = [vp, ctypes.POINTER(ScanStats)]
+        lib.blamm_hip_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+        lib.blamm_hip_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+        lib.blamm_hip_autotune.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint64, ctypes.c_uint64, vp,
+                                           ctypes.c_uint64, ctypes.c_char_p, ctypes.POINTER(Tuning),
+                                           ctypes.POINTER(ctypes.c_double)]
+        lib.blamm_hip_batch_create.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(vp)]
+        lib.blamm_hip_batch_destroy.argtypes = [vp]
+        lib.blamm_hip_batch_device_bytes.argtypes = [vp]
+        lib.blamm_hip_batch_device_bytes.restype = ctypes.c_uint64
+        lib.blamm_hip_batch_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.c_uint32]
+        lib.blamm_hip_batch_query.argtypes = [vp]
+        lib.blamm_hip_batch_wait.argtypes = [vp, _u64p]
+        lib.blamm_hip_batch_hits.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(vp), _u64p]
+        lib.blamm_hip_batch_stats.argtypes = [vp, ctypes.POINTER(ScanStats)]
+        lib.blamm_hip_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+        lib.blamm_hip_server_submit.argtypes = [vp, ctypes.POINTER(Request), _u64p]
+        lib.blamm_hip_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64]
+        lib.blamm_hip_server_result.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_uint64,
+                                                ctypes.POINTER(RequestInfo)]
+        lib.blamm_hip_server_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        lib.blamm_hip_server_destroy.argtypes = [vp]
         _hip = lib
     return _hip
 
@@ -277,3 +330,158 @@ class HipScanner:
 
     def reserve(self, n: int) -> None:
         _check(self.lib.blamm_hip_reserve(self.ctx, n))
+
+    # tile shapes
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        _check(self.lib.blamm_hip_get_tuning(self.ctx, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, chunk_tiles: int, blocks_per_wave: int, kpad_positions: int) -> None:
+        t = Tuning(chunk_tiles, blocks_per_wave, kpad_positions)
+        _check(self.lib.blamm_hip_set_tuning(self.ctx, ctypes.byref(t)))
+
+    def autotune(self, d_codes, d_lower, nbases: int, d_runs, nruns: int,
+                 cache_path: Optional[str] = None) -> Tuple[Tuning, float]:
+        """Pick the tile shapes on a device-resident sample (torch tensors)."""
+        t = Tuning()
+        ms = ctypes.c_double()
+        _check(self.lib.blamm_hip_autotune(
+            self.ctx, d_codes.data_ptr(), d_codes.numel(),
+            d_lower.data_ptr() if d_lower is not None else None, d_lower.numel() if d_lower is not None else 0,
+            nbases, d_runs.data_ptr() if nruns else None, nruns,
+            cache_path.encode() if cache_path else None, ctypes.byref(t), ctypes.byref(ms)))
+        return t, ms.value
+
+
+class ScanRequest:
+    """One stream and its thresholds, kept alive while the engine uses it."""
+
+    def __init__(self, codes: np.ndarray, nbases: int, run_starts=None, lower: Optional[np.ndarray] = None,
+                 thresholds: Optional[np.ndarray] = None, expected_hits: int = 0):
+        nw = (nbases + 31) // 32
+        self.codes = np.zeros(max(nw, 1), dtype=np.uint64)
+        self.codes[:min(nw, len(codes))] = codes[:nw]
+        self.lower = None
+        if lower is not None:
+            mw = (nbases + 63) // 64
+            self.lower = np.zeros(max(mw, 1), dtype=np.uint64)
+            self.lower[:min(mw, len(lower))] = lower[:mw]
+        self.runs = np.ascontiguousarray(run_starts if run_starts is not None else [], dtype=np.uint64)
+        self.thresholds = None if thresholds is None else np.ascontiguousarray(thresholds, dtype=np.float32)
+        self.nbases = int(nbases)
+        self.c = Request(self.codes.ctypes.data, self.lower.ctypes.data if self.lower is not None else None,
+                         self.nbases, self.runs.ctypes.data if len(self.runs) else None, len(self.runs),
+                         self.thresholds.ctypes.data if self.thresholds is not None else None, int(expected_hits))
+
+
+def _hits_array(ptr: int, n: int) -> np.ndarray:
+    if n == 0:
+        return np.zeros(0, dtype=HIT_DTYPE)
+    buf = (ctypes.c_char * (n * HIT_DTYPE.itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype=HIT_DTYPE).copy()
+
+
+class Batch:
+    """A batch slot: many requests, one captured launch sequence."""
+
+    def __init__(self, scanner: HipScanner, max_bases: int = 1 << 26, max_requests: int = 256):
+        self.scanner = scanner
+        self.lib = scanner.lib
+        self.b = ctypes.c_void_p()
+        _check(self.lib.blamm_hip_batch_create(scanner.ctx, max_bases, max_requests, ctypes.byref(self.b)))
+        self.nreq = 0
+
+    def close(self) -> None:
+        if self.b:
+            self.lib.blamm_hip_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, reqs) -> None:
+        arr = (Request * len(reqs))(*[r.c for r in reqs])
+        _check(self.lib.blamm_hip_batch_submit(self.b, arr, len(reqs)))
+        self.nreq = len(reqs)
+
+    def done(self) -> bool:
+        rc = self.lib.blamm_hip_batch_query(self.b)
+        if rc < 0:
+            _check(rc)
+        return rc == 0
+
+    def wait(self) -> int:
+        n = ctypes.c_uint64()
+        _check(self.lib.blamm_hip_batch_wait(self.b, ctypes.byref(n)))
+        return int(n.value)
+
+    def hits(self, req: int) -> np.ndarray:
+        p = ctypes.c_void_p()
+        n = ctypes.c_uint64()
+        _check(self.lib.blamm_hip_batch_hits(self.b, req, ctypes.byref(p), ctypes.byref(n)))
+        return _hits_array(p.value or 0, int(n.value))
+
+    def stats(self) -> ScanStats:
+        st = ScanStats()
+        _check(self.lib.blamm_hip_batch_stats(self.b, ctypes.byref(st)))
+        return st
+
+    def device_bytes(self) -> int:
+        return int(self.lib.blamm_hip_batch_device_bytes(self.b))
+
+
+class Server:
+    """Continuous-batching server over the scanner's resident library."""
+
+    def __init__(self, scanner: HipScanner, hbm_budget_bytes: int = 0, slots: int = 0, max_batch_requests: int = 0):
+        self.scanner = scanner
+        self.lib = scanner.lib
+        self.s = ctypes.c_void_p()
+        cfg = ServerConfig(hbm_budget_bytes, slots, max_batch_requests)
+        _check(self.lib.blamm_hip_server_create(scanner.ctx, ctypes.byref(cfg), ctypes.byref(self.s)))
+        self._keep = {}
+
+    def submit(self, req: ScanRequest) -> int:
+        i = ctypes.c_uint64()
+        _check(self.lib.blamm_hip_server_submit(self.s, ctypes.byref(req.c), ctypes.byref(i)))
+        self._keep[i.value] = req
+        return int(i.value)
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> bool:
+        rc = self.lib.blamm_hip_server_wait(self.s, rid, timeout_ms)
+        if rc < 0:
+            _check(rc)
+        return rc == 0
+
+    def result(self, rid: int) -> Tuple[np.ndarray, RequestInfo]:
+        info = RequestInfo()
+        rc = self.lib.blamm_hip_server_result(self.s, rid, None, 0, ctypes.byref(info))
+        if rc == BLAMM_ERR_OVERFLOW:
+            hits = np.zeros(info.nhits, dtype=HIT_DTYPE)
+            rc = self.lib.blamm_hip_server_result(self.s, rid, hits.ctypes.data, info.nhits, ctypes.byref(info))
+        else:
+            hits = np.zeros(0, dtype=HIT_DTYPE)
+        self._keep.pop(rid, None)
+        _check(rc)
+        return hits, info
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(self.lib.blamm_hip_server_stats(self.s, ctypes.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if self.s:
+            self.lib.blamm_hip_server_destroy(self.s)
+            self.s = ctypes.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,447 @@
+// Batch slots: many requests (streams of uneven length, each with its own
+// thresholds and column subset) scored by one persistent filter launch over
+// an explicit work list, then one finalize launch.  The launch sequence is
+// captured once into a hipGraph per slot and replayed for every batch: the
+// kernels read the batch's sizes from a device descriptor, so only buffer
+// growth or a library relayout forces a new capture.
+//
+// Included once, at the end of blamm_hip.hip (same translation unit as the
+// kernels and the library layout).
+
+#include <mutex>
+
+namespace {
+
+constexpr uint64_t kReqAlign = 64;          // bases; keeps both packed and mask words aligned
+constexpr uint64_t kBatchMaxBases = (1ull << 32) - 4096;
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+struct blamm_hip_batch {
+    blamm_hip_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_done = nullptr;
+
+    // device buffers
+    DevBuf<uint64_t> packed, lcmask, runs;
+    DevBuf<float> tab;              // [nreq][4][nslots]: filter thr, thr, seq lo, seq hi
+    DevBuf<uint32_t> req_end;
+    DevBuf<uint2> items, cand;
+    DevBuf<uint4> chunks;
+    DevBuf<BatchDesc> desc;
+    DevBuf<unsigned long long> counters;
+    DevBuf<blamm_hit> hits;
+    uint64_t buf_gen = 1;           // bumped whenever a device buffer moves
+
+    // pinned staging
+    HostBuf<uint8_t> stage;
+    HostBuf<unsigned long long> cnt_h;
+    HostBuf<blamm_hit> hits_h;
+
+    // captured launch sequence per lowercase variant
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    uint64_t exec_buf_gen[2] = {0, 0}, exec_lib_gen[2] = {0, 0};
+
+    // current batch
+    bool pending = false;
+    bool mask = false;
+    bool done = false;
+    uint32_t nreq = 0;
+    std::vector<uint64_t> req_start, req_n;
+    std::vector<blamm_hit> out;
+    std::vector<uint64_t> out_off;
+    blamm_scan_stats stats{};
+};
+
+namespace {
+
+void drop_graphs(blamm_hip_batch* b)
+{
+    for (int i = 0; i < 2; ++i) {
+        if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
+        b->exec[i] = nullptr;
+    }
+}
+
+// Grow a device buffer; a move invalidates the captured graphs.
+template <typename T>
+int batch_grow(blamm_hip_batch* b, DevBuf<T>& buf, size_t want)
+{
+    if (want <= buf.n && buf.p) return BLAMM_OK;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(buf.reserve(want + want / 8));
+    b->buf_gen++;
+    return BLAMM_OK;
+}
+
+int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
+{
+    blamm_hip_ctx* c = b->ctx;
+    const bool bf = c->precision == BLAMM_PREC_BF16X2;
+    const size_t ns = (size_t)c->nslots;
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, b->stream, b->counters.p);
+    HIP_TRY(hipGetLastError());
+    const FilterArgs a = filter_args(c, b->packed.p, mask ? b->lcmask.p : nullptr, b->tab.p, b->cand.p, b->cand.n,
+                                     b->counters.p);
+    const WorkQueue q{0, 0, 1, 1};
+    const ListArgs l{b->desc.p, b->items.p, b->chunks.p};
+    hipError_t e = launch_filter(bf, mask, true, a, q, l, grid, b->stream);
+    if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch filter launch: ") + hipGetErrorString(e));
+    FinalizeArgs f = finalize_args(c, b->cand.p, b->cand.n, b->counters.p, b->packed.p, mask ? b->lcmask.p : nullptr,
+                                   b->runs.p);
+    f.slot_thr = b->tab.p + ns;
+    f.slot_seq_lo = b->tab.p + 2 * ns;
+    f.slot_seq_hi = b->tab.p + 3 * ns;
+    f.hits = b->hits.p;
+    f.hit_cap = b->hits.n;
+    f.desc = b->desc.p;
+    f.req_end = b->req_end.p;
+    e = launch_finalize(bf, true, f, (uint32_t)c->num_cu * 8, b->stream);
+    if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch finalize launch: ") + hipGetErrorString(e));
+    return BLAMM_OK;
+}
+
+// Launch the captured sequence (capturing it first when buffers or the
+// library moved since the last capture), then copy the counters back.
+int batch_launch(blamm_hip_batch* b)
+{
+    blamm_hip_ctx* c = b->ctx;
+    const int v = b->mask ? 1 : 0;
+    if (!b->exec[v] || b->exec_buf_gen[v] != b->buf_gen || b->exec_lib_gen[v] != c->lib_gen) {
+        if (b->exec[v]) (void)hipGraphExecDestroy(b->exec[v]);
+        b->exec[v] = nullptr;
+        const bool bf = c->precision == BLAMM_PREC_BF16X2;
+        const uint32_t grid = (uint32_t)c->num_cu * (uint32_t)persistent_wgs_per_cu(bf, b->mask, true);
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_enqueue_kernels(b, b->mask, grid);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != BLAMM_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch capture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&b->exec[v], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            b->exec[v] = nullptr;
+            return fail(BLAMM_ERR_DEVICE, std::string("batch graph instantiate: ") + hipGetErrorString(ei));
+        }
+        b->exec_buf_gen[v] = b->buf_gen;
+        b->exec_lib_gen[v] = c->lib_gen;
+    }
+    HIP_TRY(hipEventRecord(b->ev_start, b->stream));
+    HIP_TRY(hipGraphLaunch(b->exec[v], b->stream));
+    HIP_TRY(hipEventRecord(b->ev_end, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->cnt_h.p, b->counters.p, kCntSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           b->stream));
+    HIP_TRY(hipEventRecord(b->ev_done, b->stream));
+    return BLAMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int blamm_hip_batch_create(blamm_hip_ctx* c, uint64_t max_bases, uint32_t max_requests, blamm_hip_batch** out)
+{
+    if (!c || !out) return fail(BLAMM_ERR_ARG, "batch_create: null pointer");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(c->device));
+    blamm_hip_batch* b = new blamm_hip_batch();
+    b->ctx = c;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev_end);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming);
+    if (e == hipSuccess) e = b->counters.reserve(kCntSlots);
+    if (e == hipSuccess) e = b->desc.reserve(1);
+    if (e == hipSuccess) e = b->cnt_h.reserve(kCntSlots);
+    const uint64_t nb = std::min<uint64_t>(max_bases, kBatchMaxBases);
+    const uint64_t nr = std::max<uint32_t>(max_requests, 1);
+    if (e == hipSuccess) e = b->packed.reserve(nb / 32 + nr * 2 + 16);
+    if (e == hipSuccess) e = b->cand.reserve(std::max<uint64_t>(nb / 256, 1u << 16));
+    if (e == hipSuccess) e = b->hits.reserve(std::max<uint64_t>(nb / 256, 1u << 16));
+    if (e != hipSuccess) {
+        blamm_hip_batch_destroy(b);
+        return fail(e == hipErrorOutOfMemory ? BLAMM_ERR_NOMEM : BLAMM_ERR_DEVICE,
+                    std::string("batch_create: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return BLAMM_OK;
+}
+
+int blamm_hip_batch_destroy(blamm_hip_batch* b)
+{
+    if (!b) return BLAMM_OK;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    b->packed.release(); b->lcmask.release(); b->runs.release(); b->tab.release(); b->req_end.release();
+    b->items.release(); b->cand.release(); b->chunks.release(); b->desc.release(); b->counters.release();
+    b->hits.release();
+    b->stage.release(); b->cnt_h.release(); b->hits_h.release();
+    if (b->ev_start) (void)hipEventDestroy(b->ev_start);
+    if (b->ev_end) (void)hipEventDestroy(b->ev_end);
+    if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return BLAMM_OK;
+}
+
+uint64_t blamm_hip_batch_device_bytes(const blamm_hip_batch* b)
+{
+    if (!b) return 0;
+    return b->packed.n * 8 + b->lcmask.n * 8 + b->runs.n * 8 + b->tab.n * 4 + b->req_end.n * 4 + b->items.n * 8 +
+           b->cand.n * 8 + b->chunks.n * 16 + b->hits.n * sizeof(blamm_hit) + sizeof(BatchDesc) +
+           kCntSlots * 8;
+}
+
+int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32_t nreq)
+{
+    if (!b) return fail(BLAMM_ERR_ARG, "batch_submit: null batch");
+    blamm_hip_ctx* c = b->ctx;
+    if (!c->have_lib) return fail(BLAMM_ERR_STATE, "batch_submit: no library uploaded");
+    if (b->pending) return fail(BLAMM_ERR_STATE, "batch_submit: previous batch not waited for");
+    if (nreq == 0 || !reqs) return fail(BLAMM_ERR_ARG, "batch_submit: no requests");
+    if (nreq >= (1u << 16)) return fail(BLAMM_ERR_ARG, "batch_submit: at most 65535 requests per batch");
+    if (c->nslots >= (1 << 16)) return fail(BLAMM_ERR_ARG, "batch_submit: library too large for batches (>= 65536 slots)");
+    HIP_TRY(hipSetDevice(c->device));
+
+    // Layout of the batch stream: request r occupies [start_r, start_r + n_r),
+    // starts aligned to kReqAlign bases.
+    b->req_start.assign(nreq, 0);
+    b->req_n.assign(nreq, 0);
+    bool mask = false;
+    uint64_t total = 0, nruns = 0;
+    for (uint32_t r = 0; r < nreq; ++r) {
+        const blamm_request& q = reqs[r];
+        if (q.nbases > 0 && !q.packed) return fail(BLAMM_ERR_ARG, "batch_submit: request " + std::to_string(r) + " has no packed stream");
+        if (q.nruns > 0 && !q.run_starts) return fail(BLAMM_ERR_ARG, "batch_submit: request " + std::to_string(r) + " has null run starts");
+        for (uint64_t i = 1; i < q.nruns; ++i)
+            if (q.run_starts[i] < q.run_starts[i - 1])
+                return fail(BLAMM_ERR_ARG, "batch_submit: run starts of request " + std::to_string(r) + " not sorted");
+        if (q.thresholds)
+            for (int j = 0; j < c->ncols; ++j)
+                if (std::isnan(q.thresholds[j]))
+                    return fail(BLAMM_ERR_ARG, "batch_submit: NaN threshold in request " + std::to_string(r));
+        b->req_start[r] = total;
+        b->req_n[r] = q.nbases;
+        total = align_up(total + q.nbases, kReqAlign);
+        if (total > kBatchMaxBases) return fail(BLAMM_ERR_ARG, "batch_submit: batch longer than 2^32 - 4096 bases");
+        mask = mask || (q.lcmask != nullptr && q.nbases > 0);
+        nruns += q.nruns + 1;
+    }
+    const uint64_t pw = total / 32 + 16, mw = mask ? total / 64 + 16 : 0;
+    const size_t nslots = (size_t)c->nslots;
+    const int cpb = c->cols_per_blk;
+    const uint32_t ct = std::max<uint32_t>(1, c->tuning.chunk_tiles);
+
+    // Active groups and thresholds per request.
+    std::vector<std::vector<uint32_t>> active(nreq);
+    std::vector<float> tabh(4 * (size_t)nreq * nslots, INFINITY);
+    for (uint32_t r = 0; r < nreq; ++r) {
+        const float* thr = reqs[r].thresholds ? reqs[r].thresholds : c->lib_thr.data();
+        float* t0 = &tabh[((size_t)r * 4 + 0) * nslots];
+        float* t1 = &tabh[((size_t)r * 4 + 1) * nslots];
+        float* t2 = &tabh[((size_t)r * 4 + 2) * nslots];
+        float* t3 = &tabh[((size_t)r * 4 + 3) * nslots];
+        for (size_t s = 0; s < nslots; ++s) {
+            const int col = c->slot_col_h[s];
+            if (col < 0 || thr[col] == INFINITY) continue;
+            const SlotThr st = slot_thresholds(thr[col], c->band_h[s], c->seq_h[s]);
+            t0[s] = st.lo;
+            t1[s] = st.thr;
+            t2[s] = st.seq_lo;
+            t3[s] = st.seq_hi;
+        }
+        for (uint32_t g = 0; g < (uint32_t)c->groups.size(); ++g) {
+            const GroupInfo& gi = c->groups[g];
+            bool any = false;
+            for (size_t s = (size_t)gi.blk0 * cpb; s < (size_t)(gi.blk0 + gi.nb) * cpb && !any; ++s)
+                any = t0[s] != INFINITY;
+            if (any) active[r].push_back(g);
+        }
+    }
+    // Chunks and items, chunk k to sub-queue k % kXcds, chunk-major.
+    std::vector<uint4> chunks;
+    std::vector<std::vector<uint2>> sub(kXcds);
+    for (uint32_t r = 0; r < nreq; ++r) {
+        if (b->req_n[r] == 0 || active[r].empty()) continue;
+        const uint32_t tb = (uint32_t)(b->req_start[r] / kBf16Tile);
+        const uint32_t te = (uint32_t)((b->req_start[r] + b->req_n[r] + kBf16Tile - 1) / kBf16Tile);
+        for (uint32_t t = tb; t < te; t += ct) {
+            const uint32_t k = (uint32_t)chunks.size();
+            chunks.push_back(make_uint4(t, std::min(te, t + ct), r, 0));
+            for (uint32_t g : active[r]) sub[k % kXcds].push_back(make_uint2(k, g));
+        }
+    }
+    BatchDesc d{};
+    size_t nitems = 0;
+    for (int x = 0; x < kXcds; ++x) {
+        d.sub_begin[x] = (uint32_t)nitems;
+        nitems += sub[x].size();
+    }
+    d.sub_begin[kXcds] = (uint32_t)nitems;
+    if (nitems >= (1ull << 32)) return fail(BLAMM_ERR_ARG, "batch_submit: work list too long");
+    d.nreq = nreq;
+    d.nbases = (uint32_t)total;
+    d.nruns = (uint32_t)nruns;
+
+    // Device capacity (growth invalidates captured graphs).
+    BLAMM_TRY(batch_grow(b, b->packed, pw));
+    if (mask) BLAMM_TRY(batch_grow(b, b->lcmask, mw));
+    BLAMM_TRY(batch_grow(b, b->runs, nruns + 1));
+    BLAMM_TRY(batch_grow(b, b->tab, tabh.size()));
+    BLAMM_TRY(batch_grow(b, b->req_end, nreq));
+    BLAMM_TRY(batch_grow(b, b->items, std::max<size_t>(nitems, 1)));
+    BLAMM_TRY(batch_grow(b, b->chunks, std::max<size_t>(chunks.size(), 1)));
+
+    // Stage everything in pinned memory, then one copy per buffer.
+    const size_t o_pk = 0;
+    const size_t o_mk = align_up(o_pk + pw * 8, 256);
+    const size_t o_rn = align_up(o_mk + mw * 8, 256);
+    const size_t o_tb = align_up(o_rn + (nruns + 1) * 8, 256);
+    const size_t o_re = align_up(o_tb + tabh.size() * 4, 256);
+    const size_t o_ch = align_up(o_re + (size_t)nreq * 4, 256);
+    const size_t o_it = align_up(o_ch + chunks.size() * 16, 256);
+    const size_t o_ds = align_up(o_it + nitems * 8, 256);
+    const size_t o_end = o_ds + sizeof(BatchDesc);
+    if (o_end > b->stage.n) {
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        HIP_TRY(b->stage.reserve(o_end + o_end / 8));
+    }
+    uint8_t* S = b->stage.p;
+    uint64_t* pk = (uint64_t*)(S + o_pk);
+    uint64_t* mk = (uint64_t*)(S + o_mk);
+    uint64_t* rn = (uint64_t*)(S + o_rn);
+    uint32_t* re = (uint32_t*)(S + o_re);
+    std::memset(pk, 0, pw * 8);
+    if (mask) std::memset(mk, 0, mw * 8);
+    size_t ir = 0;
+    for (uint32_t r = 0; r < nreq; ++r) {
+        const blamm_request& q = reqs[r];
+        const uint64_t s0 = b->req_start[r];
+        if (q.nbases > 0) {
+            std::memcpy(pk + s0 / 32, q.packed, ((q.nbases + 31) / 32) * 8);
+            if (q.lcmask) std::memcpy(mk + s0 / 64, q.lcmask, ((q.nbases + 63) / 64) * 8);
+        }
+        rn[ir++] = s0;
+        for (uint64_t i = 0; i < q.nruns; ++i)
+            if (q.run_starts[i] > 0 && q.run_starts[i] < q.nbases) rn[ir++] = s0 + q.run_starts[i];
+        re[r] = (uint32_t)(s0 + q.nbases);
+    }
+    d.nruns = (uint32_t)ir;
+    std::memcpy(S + o_tb, tabh.data(), tabh.size() * 4);
+    std::memcpy(S + o_ch, chunks.data(), chunks.size() * 16);
+    size_t at = 0;
+    for (int x = 0; x < kXcds; ++x) {
+        std::memcpy(S + o_it + at * 8, sub[x].data(), sub[x].size() * 8);
+        at += sub[x].size();
+    }
+    std::memcpy(S + o_ds, &d, sizeof(d));
+
+    hipStream_t s = b->stream;
+    HIP_TRY(hipMemcpyAsync(b->packed.p, S + o_pk, pw * 8, hipMemcpyHostToDevice, s));
+    if (mask) HIP_TRY(hipMemcpyAsync(b->lcmask.p, S + o_mk, mw * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->runs.p, S + o_rn, std::max<size_t>(ir, 1) * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->tab.p, S + o_tb, tabh.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->req_end.p, S + o_re, (size_t)nreq * 4, hipMemcpyHostToDevice, s));
+    if (!chunks.empty()) HIP_TRY(hipMemcpyAsync(b->chunks.p, S + o_ch, chunks.size() * 16, hipMemcpyHostToDevice, s));
+    if (nitems) HIP_TRY(hipMemcpyAsync(b->items.p, S + o_it, nitems * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->desc.p, S + o_ds, sizeof(BatchDesc), hipMemcpyHostToDevice, s));
+
+    b->mask = mask;
+    b->nreq = nreq;
+    b->done = false;
+    BLAMM_TRY(batch_launch(b));
+    b->pending = true;
+    return BLAMM_OK;
+}
+
+int blamm_hip_batch_query(blamm_hip_batch* b)
+{
+    if (!b) return fail(BLAMM_ERR_ARG, "batch_query: null batch");
+    if (!b->pending) return BLAMM_OK;
+    const hipError_t e = hipEventQuery(b->ev_done);
+    if (e == hipSuccess) return BLAMM_OK;
+    if (e == hipErrorNotReady) return 1;
+    return fail(BLAMM_ERR_DEVICE, std::string("batch_query: ") + hipGetErrorString(e));
+}
+
+int blamm_hip_batch_wait(blamm_hip_batch* b, uint64_t* nhits)
+{
+    if (!b) return fail(BLAMM_ERR_ARG, "batch_wait: null batch");
+    if (!b->pending) return fail(BLAMM_ERR_STATE, "batch_wait: no batch in flight");
+    blamm_hip_ctx* c = b->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    unsigned long long cand = 0, nh = 0;
+    for (int attempt = 0;; ++attempt) {
+        hipError_t e = hipEventSynchronize(b->ev_done);
+        if (e != hipSuccess) {
+            b->pending = false;
+            return fail(BLAMM_ERR_DEVICE, std::string("batch_wait: ") + hipGetErrorString(e));
+        }
+        cand = b->cnt_h.p[kCntCand];
+        nh = b->cnt_h.p[kCntHits];
+        if (cand <= b->cand.n && nh <= b->hits.n) break;
+        if (attempt >= 4) {
+            b->pending = false;
+            return fail(BLAMM_ERR_DEVICE, "batch_wait: capacity retries exhausted");
+        }
+        // Grow what overflowed and replay the batch (its inputs are still on
+        // the device).
+        if (cand > b->cand.n) BLAMM_TRY(batch_grow(b, b->cand, cand + cand / 4 + 1024));
+        if (nh > b->hits.n) BLAMM_TRY(batch_grow(b, b->hits, nh + nh / 4 + 1024));
+        BLAMM_TRY(batch_launch(b));
+    }
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev_start, b->ev_end));
+    b->stats.candidates = cand;
+    b->stats.band_rescored = b->cnt_h.p[kCntBand];
+    b->stats.hits = nh;
+    b->stats.kernel_ms = ms;
+    b->stats.filter_ms = 0.f;
+    if (nh > b->hits_h.n) HIP_TRY(b->hits_h.reserve(nh + nh / 8));
+    if (nh > 0) {
+        HIP_TRY(hipMemcpyAsync(b->hits_h.p, b->hits.p, nh * sizeof(blamm_hit), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+    }
+    // Group by request; positions become request-local.
+    b->out_off.assign((size_t)b->nreq + 1, 0);
+    for (uint64_t i = 0; i < nh; ++i) b->out_off[(b->hits_h.p[i].pos >> 32) + 1]++;
+    for (uint32_t r = 0; r < b->nreq; ++r) b->out_off[r + 1] += b->out_off[r];
+    b->out.resize(nh);
+    std::vector<uint64_t> fill(b->out_off.begin(), b->out_off.end() - 1);
+    for (uint64_t i = 0; i < nh; ++i) {
+        blamm_hit h = b->hits_h.p[i];
+        const uint32_t r = (uint32_t)(h.pos >> 32);
+        h.pos = (h.pos & 0xFFFFFFFFull) - b->req_start[r];
+        b->out[fill[r]++] = h;
+    }
+    b->pending = false;
+    b->done = true;
+    if (nhits) *nhits = nh;
+    return BLAMM_OK;
+}
+
+int blamm_hip_batch_hits(blamm_hip_batch* b, uint32_t req, const blamm_hit** hits, uint64_t* n)
+{
+    if (!b || !hits || !n) return fail(BLAMM_ERR_ARG, "batch_hits: null pointer");
+    if (!b->done) return fail(BLAMM_ERR_STATE, "batch_hits: no finished batch");
+    if (req >= b->nreq) return fail(BLAMM_ERR_ARG, "batch_hits: request index out of range");
+    *hits = b->out.data() + b->out_off[req];
+    *n = b->out_off[req + 1] - b->out_off[req];
+    return BLAMM_OK;
+}
+
+int blamm_hip_batch_stats(blamm_hip_batch* b, blamm_scan_stats* st)
+{
+    if (!b || !st) return fail(BLAMM_ERR_ARG, "batch_stats: null pointer");
+    *st = b->stats;
+    return BLAMM_OK;
+}
+
+}  // extern "C"

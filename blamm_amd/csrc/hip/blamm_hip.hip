@@ -3,15 +3,18 @@
 // Replaces the reference's HAVE_CUDA scan worker (scanThreadCUBLAS,
 // pwmscan.cpp:297-437) and kernel.cu.  Host work here is limited to laying the
 // library out for the MFMA kernels (column blocks, fragment swizzle, bf16
-// split, proven filter bands) and to moving buffers; scoring happens on the
-// device in scan_kernels.hpp.
+// split, proven filter bands), to building work lists and to moving buffers;
+// scoring happens on the device in scan_kernels.hpp.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -38,15 +41,15 @@ int fail(int code, const std::string& msg)
                         std::string(#expr) + ": " + hipGetErrorString(e_));                   \
     } while (0)
 
-// Column blocks held in registers per wave.
-constexpr int kNbBf16 = 2;
-constexpr int kNbFp32 = 4;
+#define BLAMM_TRY(expr)                \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != BLAMM_OK) return rc_; \
+    } while (0)
 
-struct ClassInfo {
-    int nch;          // chunks of 4 positions
-    int blk_begin;    // first device block
-    int nblk;         // blocks (multiple of NB)
-};
+// Column blocks one wave holds in registers at most.
+constexpr int kNbMaxBf16 = 2;
+constexpr int kNbMaxFp32 = 4;
 
 template <typename T>
 struct DevBuf {
@@ -64,6 +67,27 @@ struct DevBuf {
         if (want <= n && p) return hipSuccess;
         release();
         hipError_t e = hipMalloc((void**)&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = std::max<size_t>(want, 1);
+        return e;
+    }
+};
+
+// Pinned host staging (hipHostMalloc; memory is never registered after the fact).
+template <typename T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t reserve(size_t want)
+    {
+        if (want <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(want, 1) * sizeof(T), hipHostMallocDefault);
         if (e == hipSuccess) n = std::max<size_t>(want, 1);
         return e;
     }
@@ -103,27 +127,56 @@ float round_up(double d)
 // gamma_n = n u / (1 - n u): bound factor for any summation order of n terms.
 double gamma_n(double n, double u) { return n * u / (1.0 - n * u); }
 
+// Per-slot thresholds as the kernels use them, from one column threshold.
+struct SlotThr {
+    float lo, thr, seq_lo, seq_hi;
+};
+
+SlotThr slot_thresholds(float t, double band, double seq)
+{
+    SlotThr s;
+    s.thr = t;
+    s.lo = round_down((double)t - band);
+    s.seq_lo = round_down((double)t - seq);
+    s.seq_hi = round_up((double)t + seq);
+    return s;
+}
+
+struct GroupInfo {
+    uint32_t blk0, nch, nb;
+};
+
 }  // namespace
 
 struct blamm_hip_ctx {
     int device = 0;
     int num_cu = 256;
+    std::string arch;
     int filter_rounds = 1;      // resident-wave multiples of the persistent grid
-    uint32_t chunk_tiles = 512; // tiles per work-queue item
+    blamm_tuning tuning{512, 2, 4};
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
-    // resident library
+    // library as given (kept for relayout under another tuning)
     bool have_lib = false;
     int precision = 0;
     int ncols = 0;
+    int ldp = 0;
+    std::vector<float> lib_P, lib_thr;
+    std::vector<int> lib_len;
+    uint64_t lib_gen = 0;       // bumped by every relayout
+
+    // device layout of the library
     int nslots = 0;
-    std::vector<ClassInfo> classes;
+    int cols_per_blk = 32;
+    std::vector<GroupInfo> groups;             // longest motifs first
+    std::vector<int> slot_col_h;               // -1 = empty slot
+    std::vector<double> band_h, seq_h;         // per slot
     double max_filter_band = 0.0, max_score_error = 0.0;
     DevBuf<uint8_t> frag_a, frag_b;
     DevBuf<float> thr_lo, slot_thr, slot_seq_lo, slot_seq_hi, slot_w;
     DevBuf<int> slot_len;
-    DevBuf<uint32_t> slot_col;
+    DevBuf<uint32_t> slot_col, groups_d;
 
     // scratch
     DevBuf<uint2> cand;
@@ -142,36 +195,54 @@ struct blamm_hip_ctx {
 
 namespace {
 
-template <bool BF, bool MASK>
-const void* persistent_kernel()
+template <bool BF, bool MASK, bool LIST>
+const void* filter_kernel_ptr()
 {
-    return BF ? (const void*)filter_persistent_kernel<true, kNbBf16, MASK>
-              : (const void*)filter_persistent_kernel<false, kNbFp32, MASK>;
+    return (const void*)filter_persistent_kernel<BF, MASK, LIST>;
+}
+
+const void* filter_kernel(bool bf, bool mask, bool list)
+{
+    if (bf) {
+        if (mask) return list ? filter_kernel_ptr<true, true, true>() : filter_kernel_ptr<true, true, false>();
+        return list ? filter_kernel_ptr<true, false, true>() : filter_kernel_ptr<true, false, false>();
+    }
+    if (mask) return list ? filter_kernel_ptr<false, true, true>() : filter_kernel_ptr<false, true, false>();
+    return list ? filter_kernel_ptr<false, false, true>() : filter_kernel_ptr<false, false, false>();
 }
 
 // Workgroups of the persistent filter resident per CU (register/LDS limits).
-int persistent_wgs_per_cu(bool bf, bool mask)
+int persistent_wgs_per_cu(bool bf, bool mask, bool list)
 {
     int n = 0;
-    hipError_t e;
-    if (bf)
-        e = mask ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_persistent_kernel<true, kNbBf16, true>, kThreads, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_persistent_kernel<true, kNbBf16, false>, kThreads, 0);
-    else
-        e = mask ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_persistent_kernel<false, kNbFp32, true>, kThreads, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_persistent_kernel<false, kNbFp32, false>, kThreads, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_kernel(bf, mask, list), kThreads, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-hipError_t launch_persistent(bool bf, bool mask, const FilterArgs& a, const WorkQueue& q, uint32_t grid,
-                             hipStream_t s)
+hipError_t launch_filter(bool bf, bool mask, bool list, const FilterArgs& a, const WorkQueue& q, const ListArgs& l,
+                         uint32_t grid, hipStream_t s)
+{
+#define BLAMM_LAUNCH(B, M, L) \
+    hipLaunchKernelGGL((filter_persistent_kernel<B, M, L>), dim3(grid), dim3(kThreads), 0, s, a, q, l)
+    if (bf) {
+        if (mask) { if (list) BLAMM_LAUNCH(true, true, true); else BLAMM_LAUNCH(true, true, false); }
+        else { if (list) BLAMM_LAUNCH(true, false, true); else BLAMM_LAUNCH(true, false, false); }
+    } else {
+        if (mask) { if (list) BLAMM_LAUNCH(false, true, true); else BLAMM_LAUNCH(false, true, false); }
+        else { if (list) BLAMM_LAUNCH(false, false, true); else BLAMM_LAUNCH(false, false, false); }
+    }
+#undef BLAMM_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(bool bf, bool list, const FinalizeArgs& f, uint32_t grid, hipStream_t s)
 {
     if (bf) {
-        if (mask) hipLaunchKernelGGL((filter_persistent_kernel<true, kNbBf16, true>), dim3(grid), dim3(kThreads), 0, s, a, q);
-        else hipLaunchKernelGGL((filter_persistent_kernel<true, kNbBf16, false>), dim3(grid), dim3(kThreads), 0, s, a, q);
+        if (list) hipLaunchKernelGGL((finalize_kernel<16, true>), dim3(grid), dim3(kThreads), 0, s, f);
+        else hipLaunchKernelGGL((finalize_kernel<16, false>), dim3(grid), dim3(kThreads), 0, s, f);
     } else {
-        if (mask) hipLaunchKernelGGL((filter_persistent_kernel<false, kNbFp32, true>), dim3(grid), dim3(kThreads), 0, s, a, q);
-        else hipLaunchKernelGGL((filter_persistent_kernel<false, kNbFp32, false>), dim3(grid), dim3(kThreads), 0, s, a, q);
+        if (list) hipLaunchKernelGGL((finalize_kernel<4, true>), dim3(grid), dim3(kThreads), 0, s, f);
+        else hipLaunchKernelGGL((finalize_kernel<4, false>), dim3(grid), dim3(kThreads), 0, s, f);
     }
     return hipGetLastError();
 }
@@ -179,6 +250,250 @@ hipError_t launch_persistent(bool bf, bool mask, const FilterArgs& a, const Work
 __global__ void store_count_kernel(const unsigned long long* counters, uint64_t* out)
 {
     if (threadIdx.x == 0) *out = counters[kCntHits];
+}
+
+int check_tuning(const blamm_tuning& t, int precision)
+{
+    if (t.chunk_tiles < 1 || t.chunk_tiles > (1u << 20)) return fail(BLAMM_ERR_ARG, "tuning: chunk_tiles outside 1..2^20");
+    const uint32_t nbmax = precision == BLAMM_PREC_BF16X2 ? kNbMaxBf16 : kNbMaxFp32;
+    if (t.blocks_per_wave != 1 && t.blocks_per_wave != 2 && t.blocks_per_wave != 4)
+        return fail(BLAMM_ERR_ARG, "tuning: blocks_per_wave must be 1, 2 or 4");
+    if (t.blocks_per_wave > nbmax)
+        return fail(BLAMM_ERR_ARG, "tuning: blocks_per_wave above " + std::to_string(nbmax) + " for this precision");
+    if (t.kpad_positions != 4 && t.kpad_positions != 8)
+        return fail(BLAMM_ERR_ARG, "tuning: kpad_positions must be 4 or 8");
+    return BLAMM_OK;
+}
+
+// Lay the stored library out for the kernels under the context's tuning and
+// upload it.
+int relayout(blamm_hip_ctx* c)
+{
+    const bool bf = c->precision == BLAMM_PREC_BF16X2;
+    const int ncols = c->ncols, ldp = c->ldp;
+    const int* lengths = c->lib_len.data();
+    const float* P = c->lib_P.data();
+    const int cols_per_blk = bf ? kBf16Cols : kFp32Cols;
+    const uint32_t nbmax = std::min<uint32_t>(c->tuning.blocks_per_wave, bf ? kNbMaxBf16 : kNbMaxFp32);
+    const int kstep = (int)c->tuning.kpad_positions / 4;   // chunk granularity
+
+    // Columns by non-decreasing length (stable); consecutive runs of
+    // cols_per_blk form a block whose chunk count is set by its longest
+    // column, rounded up to the K-padding granularity.
+    std::vector<int> order(ncols);
+    for (int j = 0; j < ncols; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return lengths[x] < lengths[y]; });
+    std::vector<std::vector<int>> blocks;   // each: cols_per_blk entries, -1 = empty
+    std::vector<int> blk_nch;
+    for (int s = 0; s < ncols; s += cols_per_blk) {
+        std::vector<int> b(cols_per_blk, -1);
+        int lm = 0;
+        for (int k = 0; k < cols_per_blk && s + k < ncols; ++k) {
+            b[k] = order[s + k];
+            lm = std::max(lm, lengths[order[s + k]]);
+        }
+        blocks.push_back(b);
+        int nch = (lm + 3) / 4;
+        nch = std::min(kMaxChunks, (nch + kstep - 1) / kstep * kstep);
+        blk_nch.push_back(nch);
+    }
+    // Groups: runs of equal chunk count cut into nbmax blocks, the remainder
+    // into smaller powers of two, so no padding block is scored.
+    std::vector<GroupInfo> groups;
+    for (size_t i = 0; i < blocks.size();) {
+        size_t e = i;
+        while (e < blocks.size() && blk_nch[e] == blk_nch[i]) ++e;
+        size_t b = i;
+        for (uint32_t nb = nbmax; nb >= 1; nb /= 2)
+            while (e - b >= nb) {
+                groups.push_back({(uint32_t)b, (uint32_t)blk_nch[i], nb});
+                b += nb;
+            }
+        i = e;
+    }
+    std::stable_sort(groups.begin(), groups.end(), [](const GroupInfo& x, const GroupInfo& y) { return x.nch > y.nch; });
+    const int nblk = (int)blocks.size();
+    const int nslots = nblk * cols_per_blk;
+    if (nblk >= (1 << 20)) return fail(BLAMM_ERR_ARG, "set_library: too many columns");
+
+    std::vector<float> thr_lo(nslots, INFINITY), s_thr(nslots, INFINITY), s_lo(nslots, INFINITY),
+        s_hi(nslots, INFINITY), s_w((size_t)nslots * kMaxLen * 4, 0.f);
+    std::vector<int> s_len(nslots, 1), s_colh(nslots, -1);
+    std::vector<uint32_t> s_col(nslots, 0);
+    std::vector<double> s_band(nslots, 0.0), s_seq(nslots, 0.0);
+    std::vector<uint16_t> hi16((size_t)nslots * kMaxLen * 4, 0), lo16((size_t)nslots * kMaxLen * 4, 0);
+    double max_band = 0.0, max_err = 0.0;
+    const double u24 = std::ldexp(1.0, -24), u53 = std::ldexp(1.0, -53);
+    for (int bi = 0; bi < nblk; ++bi) {
+        const int nch = blk_nch[bi];
+        for (int k = 0; k < cols_per_blk; ++k) {
+            const int col = blocks[bi][k];
+            if (col < 0) continue;
+            const int slot = bi * cols_per_blk + k;
+            const int L = lengths[col];
+            const float* w = P + (size_t)col * ldp;
+            double sum_max = 0.0, sum_split = 0.0, rep_err = 0.0;
+            for (int p = 0; p < L; ++p) {
+                double mx = 0.0, mxs = 0.0, me = 0.0;
+                for (int b = 0; b < 4; ++b) {
+                    const float x = w[4 * p + b];
+                    s_w[((size_t)slot * kMaxLen + p) * 4 + b] = x;
+                    mx = std::max(mx, (double)std::fabs(x));
+                    if (bf) {
+                        const uint16_t h = bf16_rne(x);
+                        const float hf = bf16_to_float(h);
+                        const uint16_t l = bf16_rne(x - hf);   // x - hf is exact in FP32
+                        const float lf = bf16_to_float(l);
+                        hi16[((size_t)slot * kMaxLen + p) * 4 + b] = h;
+                        lo16[((size_t)slot * kMaxLen + p) * 4 + b] = l;
+                        me = std::max(me, std::fabs((double)x - (double)hf - (double)lf));
+                        mxs = std::max(mxs, std::fabs((double)hf) + std::fabs((double)lf));
+                    } else {
+                        mxs = std::max(mxs, (double)std::fabs(x));
+                    }
+                }
+                sum_max += mx;
+                sum_split += mxs;
+                rep_err += me;
+            }
+            // |FP64 left-to-right sum - exact sum| (binary64, L-1 additions)
+            const double fp64_err = gamma_n(L, u53) * sum_max;
+            // MFMA filter band: representation error of the split plus any
+            // summation order of every product the window's dot product sees
+            // (32 per chunk for hi+lo, 16 per chunk for FP32), with unit
+            // roundoff 2^-22 to allow for truncating accumulation.
+            const double nterms = bf ? 32.0 * nch : 16.0 * nch;
+            const double band = 2.0 * (rep_err + gamma_n(nterms, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
+            // Sequential FP32 band used by finalize (RNE adds, L-1 of them).
+            const double seq = 2.0 * (gamma_n(L, u24) * sum_max + fp64_err) + 1e-30;
+            const SlotThr st = slot_thresholds(c->lib_thr[col], band, seq);
+            thr_lo[slot] = st.lo;
+            s_thr[slot] = st.thr;
+            s_lo[slot] = st.seq_lo;
+            s_hi[slot] = st.seq_hi;
+            s_len[slot] = L;
+            s_col[slot] = (uint32_t)col;
+            s_colh[slot] = col;
+            s_band[slot] = band;
+            s_seq[slot] = seq;
+            max_band = std::max(max_band, band);
+            max_err = std::max(max_err, seq);
+        }
+    }
+
+    // Fragment swizzle.
+    std::vector<uint8_t> fa, fb;
+    if (bf) {
+        // [block][chunk][lane] uint4: lane l holds column l%32, positions
+        // 4j + 2(l/32) + {0,1}, 4 bases each, as bf16 pairs.
+        fa.assign((size_t)nblk * kMaxChunks * 64 * 16, 0);
+        fb.assign(fa.size(), 0);
+        for (int bi = 0; bi < nblk; ++bi)
+            for (int j = 0; j < kMaxChunks; ++j)
+                for (int l = 0; l < 64; ++l) {
+                    const int slot = bi * kBf16Cols + (l & 31);
+                    uint16_t vh[8], vl[8];
+                    for (int i = 0; i < 8; ++i) {
+                        const int p = 4 * j + 2 * (l >> 5) + i / 4, b = i % 4;
+                        vh[i] = hi16[((size_t)slot * kMaxLen + p) * 4 + b];
+                        vl[i] = lo16[((size_t)slot * kMaxLen + p) * 4 + b];
+                    }
+                    const size_t off = (((size_t)bi * kMaxChunks + j) * 64 + l) * 16;
+                    std::memcpy(&fa[off], vh, 16);
+                    std::memcpy(&fb[off], vl, 16);
+                }
+    } else {
+        // [block][position][lane] float: lane l holds W_{col l%16}[p][l/16].
+        fa.assign((size_t)nblk * kMaxLen * 64 * 4, 0);
+        for (int bi = 0; bi < nblk; ++bi)
+            for (int p = 0; p < kMaxLen; ++p)
+                for (int l = 0; l < 64; ++l) {
+                    const int slot = bi * kFp32Cols + (l & 15);
+                    const float x = s_w[((size_t)slot * kMaxLen + p) * 4 + (l >> 4)];
+                    std::memcpy(&fa[(((size_t)bi * kMaxLen + p) * 64 + l) * 4], &x, 4);
+                }
+        fb.assign(16, 0);
+    }
+    std::vector<uint32_t> gpk(groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) gpk[g] = group_pack(groups[g].blk0, groups[g].nch, groups[g].nb);
+
+    c->have_lib = false;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(c->frag_a.reserve(fa.size()));
+    HIP_TRY(c->frag_b.reserve(fb.size()));
+    HIP_TRY(c->thr_lo.reserve(nslots));
+    HIP_TRY(c->slot_thr.reserve(nslots));
+    HIP_TRY(c->slot_seq_lo.reserve(nslots));
+    HIP_TRY(c->slot_seq_hi.reserve(nslots));
+    HIP_TRY(c->slot_w.reserve(s_w.size()));
+    HIP_TRY(c->slot_len.reserve(nslots));
+    HIP_TRY(c->slot_col.reserve(nslots));
+    HIP_TRY(c->groups_d.reserve(gpk.size()));
+    HIP_TRY(hipMemcpy(c->frag_a.p, fa.data(), fa.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->frag_b.p, fb.data(), fb.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->thr_lo.p, thr_lo.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_thr.p, s_thr.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_seq_lo.p, s_lo.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_seq_hi.p, s_hi.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_w.p, s_w.data(), s_w.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_len.p, s_len.data(), nslots * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_col.p, s_col.data(), nslots * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->groups_d.p, gpk.data(), gpk.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->nslots = nslots;
+    c->cols_per_blk = cols_per_blk;
+    c->groups = groups;
+    c->slot_col_h = s_colh;
+    c->band_h = s_band;
+    c->seq_h = s_seq;
+    c->max_filter_band = max_band;
+    c->max_score_error = max_err;
+    c->lib_gen++;
+    c->have_lib = true;
+    return BLAMM_OK;
+}
+
+FilterArgs filter_args(const blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, const float* thr_lo,
+                       uint2* cand, uint64_t cand_cap, unsigned long long* counters)
+{
+    FilterArgs a;
+    a.packed = packed;
+    a.lcmask = lcmask;
+    a.frag_a = c->frag_a.p;
+    a.frag_b = c->frag_b.p;
+    a.thr_lo = thr_lo;
+    a.groups = c->groups_d.p;
+    a.cand = cand;
+    a.counters = counters;
+    a.cand_cap = cand_cap;
+    a.req_stride = 4u * (uint32_t)c->nslots;
+    return a;
+}
+
+FinalizeArgs finalize_args(const blamm_hip_ctx* c, const uint2* cand, uint64_t cand_cap, unsigned long long* counters,
+                           const uint64_t* packed, const uint64_t* lcmask, const uint64_t* runs)
+{
+    FinalizeArgs f;
+    f.cand = cand;
+    f.counters = counters;
+    f.counters_rw = counters;
+    f.cand_cap = cand_cap;
+    f.packed = packed;
+    f.lcmask = lcmask;
+    f.nbases = 0;
+    f.run_starts = runs;
+    f.nruns = 0;
+    f.slot_len = c->slot_len.p;
+    f.slot_w = c->slot_w.p;
+    f.slot_thr = c->slot_thr.p;
+    f.slot_seq_lo = c->slot_seq_lo.p;
+    f.slot_seq_hi = c->slot_seq_hi.p;
+    f.slot_col = c->slot_col.p;
+    f.hits = nullptr;
+    f.hit_cap = 0;
+    f.req_stride = 4u * (uint32_t)c->nslots;
+    f.desc = nullptr;
+    f.req_end = nullptr;
+    return f;
 }
 
 }  // namespace
@@ -219,10 +534,11 @@ int blamm_hip_create(int device, blamm_hip_ctx** out)
         return fail(BLAMM_ERR_DEVICE, "libblamm_hip.so is built for gfx950, device is " + arch);
     blamm_hip_ctx* c = new blamm_hip_ctx();
     c->device = device;
+    c->arch = arch.substr(0, arch.find(':'));
     c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (const char* e = std::getenv("BLAMM_CHUNK_TILES")) {
         const int v = std::atoi(e);
-        if (v >= 1 && v <= (1 << 20)) c->chunk_tiles = (uint32_t)v;
+        if (v >= 1 && v <= (1 << 20)) c->tuning.chunk_tiles = (uint32_t)v;
     }
     if (const char* e = std::getenv("BLAMM_FILTER_ROUNDS")) {
         const int v = std::atoi(e);
@@ -246,7 +562,8 @@ int blamm_hip_destroy(blamm_hip_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->frag_a.release(); c->frag_b.release(); c->thr_lo.release(); c->slot_thr.release();
     c->slot_seq_lo.release(); c->slot_seq_hi.release(); c->slot_w.release();
-    c->slot_len.release(); c->slot_col.release(); c->cand.release(); c->counters.release();
+    c->slot_len.release(); c->slot_col.release(); c->groups_d.release(); c->cand.release();
+    c->counters.release();
     c->packed.release(); c->lcmask.release(); c->runs.release(); c->nhits_dev.release();
     c->hits.release();
     for (int i = 0; i < 4; ++i)
@@ -281,170 +598,16 @@ int blamm_hip_set_library(blamm_hip_ctx* c, const float* P, int ldp, int ncols,
                                                " (PSEUDOCOUNT must keep every PWM entry finite)");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-
-    const bool bf = precision == BLAMM_PREC_BF16X2;
-    const int cols_per_blk = bf ? kBf16Cols : kFp32Cols;
-    const int nb = bf ? kNbBf16 : kNbFp32;
-
-    // Columns by non-decreasing length (stable); consecutive runs of
-    // cols_per_blk form a block whose chunk count is set by its longest column.
-    std::vector<int> order(ncols);
-    for (int j = 0; j < ncols; ++j) order[j] = j;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return lengths[x] < lengths[y]; });
-    std::vector<std::vector<int>> blocks;   // each: cols_per_blk entries, -1 = empty
-    std::vector<int> blk_nch;
-    for (int s = 0; s < ncols; s += cols_per_blk) {
-        std::vector<int> b(cols_per_blk, -1);
-        int lm = 0;
-        for (int k = 0; k < cols_per_blk && s + k < ncols; ++k) {
-            b[k] = order[s + k];
-            lm = std::max(lm, lengths[order[s + k]]);
-        }
-        blocks.push_back(b);
-        blk_nch.push_back((lm + 3) / 4);
-    }
-    // Classes of equal chunk count, each padded to a multiple of NB blocks.
-    std::vector<std::vector<int>> dev_blocks;
-    std::vector<ClassInfo> classes;
-    for (size_t i = 0; i < blocks.size();) {
-        size_t e = i;
-        while (e < blocks.size() && blk_nch[e] == blk_nch[i]) ++e;
-        ClassInfo ci;
-        ci.nch = blk_nch[i];
-        ci.blk_begin = (int)dev_blocks.size();
-        for (size_t k = i; k < e; ++k) dev_blocks.push_back(blocks[k]);
-        while ((dev_blocks.size() - ci.blk_begin) % nb) dev_blocks.push_back(std::vector<int>(cols_per_blk, -1));
-        ci.nblk = (int)dev_blocks.size() - ci.blk_begin;
-        classes.push_back(ci);
-        i = e;
-    }
-    const int nblk = (int)dev_blocks.size();
-    const int nslots = nblk * cols_per_blk;
-
-    std::vector<float> thr_lo(nslots, INFINITY), s_thr(nslots, INFINITY), s_lo(nslots, INFINITY),
-        s_hi(nslots, INFINITY), s_w((size_t)nslots * kMaxLen * 4, 0.f);
-    std::vector<int> s_len(nslots, 1);
-    std::vector<uint32_t> s_col(nslots, 0);
-    std::vector<uint16_t> hi16((size_t)nslots * kMaxLen * 4, 0), lo16((size_t)nslots * kMaxLen * 4, 0);
-    double max_band = 0.0, max_err = 0.0;
-    const double u24 = std::ldexp(1.0, -24), u53 = std::ldexp(1.0, -53);
-    for (int bi = 0; bi < nblk; ++bi) {
-        int nch = 0;
-        for (const ClassInfo& ci : classes)
-            if (bi >= ci.blk_begin && bi < ci.blk_begin + ci.nblk) nch = ci.nch;
-        for (int k = 0; k < cols_per_blk; ++k) {
-            const int col = dev_blocks[bi][k];
-            if (col < 0) continue;
-            const int slot = bi * cols_per_blk + k;
-            const int L = lengths[col];
-            const float* w = P + (size_t)col * ldp;
-            double sum_max = 0.0, sum_split = 0.0, rep_err = 0.0;
-            for (int p = 0; p < L; ++p) {
-                double mx = 0.0, mxs = 0.0, me = 0.0;
-                for (int b = 0; b < 4; ++b) {
-                    const float x = w[4 * p + b];
-                    s_w[((size_t)slot * kMaxLen + p) * 4 + b] = x;
-                    mx = std::max(mx, (double)std::fabs(x));
-                    if (bf) {
-                        const uint16_t h = bf16_rne(x);
-                        const float hf = bf16_to_float(h);
-                        const uint16_t l = bf16_rne(x - hf);   // x - hf is exact in FP32
-                        const float lf = bf16_to_float(l);
-                        hi16[((size_t)slot * kMaxLen + p) * 4 + b] = h;
-                        lo16[((size_t)slot * kMaxLen + p) * 4 + b] = l;
-                        me = std::max(me, std::fabs((double)x - (double)hf - (double)lf));
-                        mxs = std::max(mxs, std::fabs((double)hf) + std::fabs((double)lf));
-                    } else {
-                        mxs = std::max(mxs, (double)std::fabs(x));
-                    }
-                }
-                sum_max += mx;
-                sum_split += mxs;
-                rep_err += me;
-            }
-            // |FP64 left-to-right sum - exact sum| (binary64, L-1 additions)
-            const double fp64_err = gamma_n(L, u53) * sum_max;
-            // MFMA filter band: representation error of the split plus any
-            // summation order of every product the window's dot product sees
-            // (32 per chunk for hi+lo, 16 per chunk for FP32), with unit
-            // roundoff 2^-22 to allow for truncating accumulation.
-            const double nterms = bf ? 32.0 * nch : 16.0 * nch;
-            const double band = 2.0 * (rep_err + gamma_n(nterms, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
-            // Sequential FP32 band used by finalize (RNE adds, L-1 of them).
-            const double seq = 2.0 * (gamma_n(L, u24) * sum_max + fp64_err) + 1e-30;
-            const double t = (double)thresholds[col];
-            thr_lo[slot] = round_down(t - band);
-            s_thr[slot] = thresholds[col];
-            s_lo[slot] = round_down(t - seq);
-            s_hi[slot] = round_up(t + seq);
-            s_len[slot] = L;
-            s_col[slot] = (uint32_t)col;
-            max_band = std::max(max_band, band);
-            max_err = std::max(max_err, seq);
-        }
-    }
-
-    // Fragment swizzle.
-    std::vector<uint8_t> fa, fb;
-    if (bf) {
-        // [block][chunk][lane] uint4: lane l holds column l%32, positions
-        // 4j + 2(l/32) + {0,1}, 4 bases each, as bf16 pairs.
-        fa.assign((size_t)nblk * kMaxChunks * 64 * 16, 0);
-        fb.assign(fa.size(), 0);
-        for (int bi = 0; bi < nblk; ++bi)
-            for (int j = 0; j < kMaxChunks; ++j)
-                for (int l = 0; l < 64; ++l) {
-                    const int slot = bi * kBf16Cols + (l & 31);
-                    uint16_t vh[8], vl[8];
-                    for (int i = 0; i < 8; ++i) {
-                        const int p = 4 * j + 2 * (l >> 5) + i / 4, b = i % 4;
-                        vh[i] = hi16[((size_t)slot * kMaxLen + p) * 4 + b];
-                        vl[i] = lo16[((size_t)slot * kMaxLen + p) * 4 + b];
-                    }
-                    const size_t off = (((size_t)bi * kMaxChunks + j) * 64 + l) * 16;
-                    std::memcpy(&fa[off], vh, 16);
-                    std::memcpy(&fb[off], vl, 16);
-                }
-    } else {
-        // [block][position][lane] float: lane l holds W_{col l%16}[p][l/16].
-        fa.assign((size_t)nblk * kMaxLen * 64 * 4, 0);
-        for (int bi = 0; bi < nblk; ++bi)
-            for (int p = 0; p < kMaxLen; ++p)
-                for (int l = 0; l < 64; ++l) {
-                    const int slot = bi * kFp32Cols + (l & 15);
-                    const float x = s_w[((size_t)slot * kMaxLen + p) * 4 + (l >> 4)];
-                    std::memcpy(&fa[(((size_t)bi * kMaxLen + p) * 64 + l) * 4], &x, 4);
-                }
-        fb.assign(16, 0);
-    }
-
     c->have_lib = false;
-    HIP_TRY(c->frag_a.reserve(fa.size()));
-    HIP_TRY(c->frag_b.reserve(fb.size()));
-    HIP_TRY(c->thr_lo.reserve(nslots));
-    HIP_TRY(c->slot_thr.reserve(nslots));
-    HIP_TRY(c->slot_seq_lo.reserve(nslots));
-    HIP_TRY(c->slot_seq_hi.reserve(nslots));
-    HIP_TRY(c->slot_w.reserve(s_w.size()));
-    HIP_TRY(c->slot_len.reserve(nslots));
-    HIP_TRY(c->slot_col.reserve(nslots));
-    HIP_TRY(hipMemcpy(c->frag_a.p, fa.data(), fa.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->frag_b.p, fb.data(), fb.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->thr_lo.p, thr_lo.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->slot_thr.p, s_thr.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->slot_seq_lo.p, s_lo.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->slot_seq_hi.p, s_hi.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->slot_w.p, s_w.data(), s_w.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->slot_len.p, s_len.data(), nslots * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->slot_col.p, s_col.data(), nslots * sizeof(uint32_t), hipMemcpyHostToDevice));
     c->precision = precision;
     c->ncols = ncols;
-    c->nslots = nslots;
-    c->classes = classes;
-    c->max_filter_band = max_band;
-    c->max_score_error = max_err;
-    c->have_lib = true;
-    return BLAMM_OK;
+    c->ldp = ldp;
+    c->lib_P.assign(P, P + (size_t)ncols * ldp);
+    c->lib_len.assign(lengths, lengths + ncols);
+    c->lib_thr.assign(thresholds, thresholds + ncols);
+    const uint32_t nbmax = precision == BLAMM_PREC_BF16X2 ? kNbMaxBf16 : kNbMaxFp32;
+    if (c->tuning.blocks_per_wave > nbmax) c->tuning.blocks_per_wave = nbmax;
+    return relayout(c);
 }
 
 int blamm_hip_library_bounds(blamm_hip_ctx* c, double* band, double* err)
@@ -452,6 +615,28 @@ int blamm_hip_library_bounds(blamm_hip_ctx* c, double* band, double* err)
     if (!c || !c->have_lib) return fail(BLAMM_ERR_STATE, "library_bounds: no library");
     if (band) *band = c->max_filter_band;
     if (err) *err = c->max_score_error;
+    return BLAMM_OK;
+}
+
+int blamm_hip_get_tuning(blamm_hip_ctx* c, blamm_tuning* t)
+{
+    if (!c || !t) return fail(BLAMM_ERR_ARG, "get_tuning: null pointer");
+    *t = c->tuning;
+    return BLAMM_OK;
+}
+
+int blamm_hip_set_tuning(blamm_hip_ctx* c, const blamm_tuning* t)
+{
+    if (!c || !t) return fail(BLAMM_ERR_ARG, "set_tuning: null pointer");
+    if (c->pending) return fail(BLAMM_ERR_STATE, "set_tuning: a scan is in flight");
+    BLAMM_TRY(check_tuning(*t, c->have_lib ? c->precision : BLAMM_PREC_FP32));
+    const bool relay = c->have_lib && (t->blocks_per_wave != c->tuning.blocks_per_wave ||
+                                       t->kpad_positions != c->tuning.kpad_positions);
+    c->tuning = *t;
+    if (relay) {
+        HIP_TRY(hipSetDevice(c->device));
+        return relayout(c);
+    }
     return BLAMM_OK;
 }
 
@@ -492,71 +677,29 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     HIP_TRY(hipEventRecord(c->ev[0], s));
     if (nbases > 0) {
         const uint32_t ntiles = (uint32_t)((nbases + kBf16Tile - 1) / kBf16Tile);   // 32-window tiles
-        const int nb = bf ? kNbBf16 : kNbFp32;
-        FilterArgs a;
-        a.packed = d_packed;
-        a.lcmask = d_lcmask;
-        a.nbases = (uint32_t)nbases;
-        a.ntiles = ntiles;
-        a.tiles_per_wave = 0;
-        a.ngroups = 0;
-        a.nwaves = 0;
-        a.blk_begin = 0;
-        a.frag_a = c->frag_a.p;
-        a.frag_b = c->frag_b.p;
-        a.thr_lo = c->thr_lo.p;
-        a.cand = c->cand.p;
-        a.counters = c->counters.p;
-        a.cand_cap = c->cand.n;
-        // work queue: column groups numbered longest motifs first
+        const FilterArgs a = filter_args(c, d_packed, d_lcmask, c->thr_lo.p, c->cand.p, c->cand.n, c->counters.p);
         WorkQueue q{};
-        q.chunk_tiles = std::max<uint32_t>(1, std::min<uint32_t>(c->chunk_tiles, ntiles));
+        q.ntiles = ntiles;
+        q.chunk_tiles = std::max<uint32_t>(1, std::min<uint32_t>(c->tuning.chunk_tiles, ntiles));
         q.nchunks = (ntiles + q.chunk_tiles - 1) / q.chunk_tiles;
-        std::vector<ClassInfo> cls = c->classes;
-        std::sort(cls.begin(), cls.end(), [](const ClassInfo& x, const ClassInfo& y) { return x.nch > y.nch; });
-        if (cls.size() > (size_t)kMaxClasses) return fail(BLAMM_ERR_ARG, "scan: too many length classes");
-        q.nclasses = (uint32_t)cls.size();
-        uint32_t groups = 0;
-        for (size_t i = 0; i < cls.size(); ++i) {
-            q.group_base[i] = groups;
-            q.nch[i] = (uint32_t)cls[i].nch;
-            q.blk_begin[i] = (uint32_t)cls[i].blk_begin;
-            groups += (uint32_t)(cls[i].nblk / nb);
-        }
-        q.group_base[cls.size()] = groups;
-        q.ngroups = groups;
-        if ((uint64_t)((q.nchunks + kXcds - 1) / kXcds) * groups >= (1ull << 32))
+        q.ngroups = (uint32_t)c->groups.size();
+        if ((uint64_t)((q.nchunks + kXcds - 1) / kXcds) * q.ngroups >= (1ull << 32))
             return fail(BLAMM_ERR_ARG, "scan: work queue too long");
-        const uint32_t grid = (uint32_t)c->num_cu * (uint32_t)persistent_wgs_per_cu(bf, mask) * (uint32_t)c->filter_rounds;
-        hipError_t e = launch_persistent(bf, mask, a, q, grid, s);
+        const ListArgs l{nullptr, nullptr, nullptr};
+        const uint32_t grid =
+            (uint32_t)c->num_cu * (uint32_t)persistent_wgs_per_cu(bf, mask, false) * (uint32_t)c->filter_rounds;
+        hipError_t e = launch_filter(bf, mask, false, a, q, l, grid, s);
         if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("filter launch: ") + hipGetErrorString(e));
     }
     HIP_TRY(hipEventRecord(c->ev[1], s));
     {
-        FinalizeArgs f;
-        f.cand = c->cand.p;
-        f.counters = c->counters.p;
-        f.counters_rw = c->counters.p;
-        f.cand_cap = c->cand.n;
-        f.packed = d_packed;
-        f.lcmask = d_lcmask;
+        FinalizeArgs f = finalize_args(c, c->cand.p, c->cand.n, c->counters.p, d_packed, d_lcmask, d_run_starts);
         f.nbases = (uint32_t)nbases;
-        f.run_starts = d_run_starts;
         f.nruns = (uint32_t)nruns;
-        f.slot_len = c->slot_len.p;
-        f.slot_w = c->slot_w.p;
-        f.slot_thr = c->slot_thr.p;
-        f.slot_seq_lo = c->slot_seq_lo.p;
-        f.slot_seq_hi = c->slot_seq_hi.p;
-        f.slot_col = c->slot_col.p;
         f.hits = d_hits;
         f.hit_cap = cap;
-        const uint32_t grid = (uint32_t)c->num_cu * 8;
-        if (bf)
-            hipLaunchKernelGGL(finalize_kernel<16>, dim3(grid), dim3(kThreads), 0, s, f);
-        else
-            hipLaunchKernelGGL(finalize_kernel<4>, dim3(grid), dim3(kThreads), 0, s, f);
-        HIP_TRY(hipGetLastError());
+        hipError_t e = launch_finalize(bf, false, f, (uint32_t)c->num_cu * 8, s);
+        if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("finalize launch: ") + hipGetErrorString(e));
     }
     HIP_TRY(hipEventRecord(c->ev[2], s));
     if (d_nhits) {
@@ -655,3 +798,7 @@ int blamm_hip_scan(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcm
 }
 
 }  // extern "C"
+
+#include "batch.hpp"
+#include "tune.hpp"
+#include "serve.hpp"

@@ -65,19 +65,23 @@ __device__ __forceinline__ uint64_t wave_reserve(unsigned long long* counter, ui
 struct FilterArgs {
     const uint64_t* packed;      // 2-bit codes
     const uint64_t* lcmask;      // lowercase bits (MASK variants only)
-    uint32_t nbases;
-    uint32_t ntiles;             // window tiles
-    uint32_t tiles_per_wave;
-    uint32_t ngroups;            // column groups (NB blocks each) of this class
-    uint32_t nwaves;             // ngroups * window ranges
-    uint32_t blk_begin;          // first column block of this class
     const void* frag_a;          // bf16: hi fragments; fp32: weights
     const void* frag_b;          // bf16: lo fragments
-    const float* thr_lo;         // per slot: filter threshold (thr - band, rounded down)
-    uint2* cand;                 // (window, slot)
+    const float* thr_lo;         // per slot: filter threshold (thr - band, rounded down);
+                                 // list mode: request r at thr_lo + r * req_stride
+    const uint32_t* groups;      // column groups, see group_pack()
+    uint2* cand;                 // (window, slot | request << 16 in list mode)
     unsigned long long* counters;
     uint64_t cand_cap;
+    uint32_t req_stride;         // list mode: floats between two requests' tables
 };
+
+// A column group is nb consecutive blocks of one length class (nch chunks of
+// 4 positions), scored together by one wave.
+__host__ __device__ constexpr uint32_t group_pack(uint32_t blk0, uint32_t nch, uint32_t nb)
+{
+    return blk0 | (nch << 20) | (nb << 26);
+}
 
 // Per-wave staging of filter candidates in LDS.  A single global counter hit
 // by every candidate serialises on one L2 atomic unit (tens of millions of
@@ -122,6 +126,19 @@ __device__ __forceinline__ void emit_lane(const FilterArgs& a, Stage& st, bool p
         if (pred) st.buf[st.n + rank] = make_uint2(qlane, slot);
         st.n += (uint32_t)__popcll(m);
     }
+}
+
+// Wait states between matrix-core results and the VALU code that reads them.
+// The compiler counts the wait states a v_mfma result needs before a VALU
+// read; for v_mfma_f32_16x16x4_f32 on gfx950 its count proved too small now
+// and then (the FP32 filter lost a candidate in about one scan in ten).  Where
+// a read can follow the last MFMA of an accumulator closely, 16 wait states
+// are placed here and pinned by scheduling barriers.
+__device__ __forceinline__ void mfma_result_guard()
+{
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("s_nop 7\n\ts_nop 7" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
 }
 
 // 64 bits of the 128-bit value (hi:lo) starting at bit s (0 <= s < 64),
@@ -205,7 +222,8 @@ __device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const uint4* o
 // 8(r/4) + 4(l/32) + r%4, i.e. windows qlane + {0..3, 8..11, 16..19, 24..27}.
 template <int NB>
 __device__ __forceinline__ void bf16_epilogue(const FilterArgs& a, Stage& st, const f32x16 (&acc)[NB],
-                                              const float (&th)[NB], uint32_t q0, uint32_t blk0, uint32_t lane)
+                                              const float (&th)[NB], uint32_t q0, uint32_t blk0, uint32_t rtag,
+                                              uint32_t lane)
 {
     const uint32_t qlane = q0 + 4 * (lane >> 5);
 #pragma unroll
@@ -213,7 +231,7 @@ __device__ __forceinline__ void bf16_epilogue(const FilterArgs& a, Stage& st, co
         float mx = acc[b][0];
 #pragma unroll
         for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[b][r]);
-        emit_lane(a, st, mx >= th[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
+        emit_lane(a, st, mx >= th[b], qlane, ((blk0 + b) * kBf16Cols + (lane & 31)) | rtag);
     }
 }
 
@@ -230,14 +248,16 @@ __device__ __forceinline__ uint4 onehot_entry(uint32_t e)
     return make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
 }
 
-// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1.  The tile loop is
+// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1, filter thresholds thr
+// (per slot), records tagged with rtag.  The tile loop is
 // software-pipelined over two accumulator sets: the MFMAs of tile t are
 // issued before the threshold test of tile t-1, so the epilogue's VALU work
 // and its branch overlap matrix-core work of the same wave.  Packed words are
 // read two tiles ahead.
 template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const uint4* onehot, uint32_t blk0,
-                                           uint32_t t0, uint32_t t1, uint32_t lane)
+                                           const float* __restrict__ thr, uint32_t rtag, uint32_t t0, uint32_t t1,
+                                           uint32_t lane)
 {
     const uint4* __restrict__ fh = (const uint4*)a.frag_a;
     const uint4* __restrict__ fl = (const uint4*)a.frag_b;
@@ -252,7 +272,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
             bh[b][j] = __builtin_bit_cast(bf16x8, h);
             bl[b][j] = __builtin_bit_cast(bf16x8, l);
         }
-        th[b] = a.thr_lo[(blk0 + b) * kBf16Cols + (lane & 31)];
+        th[b] = thr[(blk0 + b) * kBf16Cols + (lane & 31)];
     }
 
     const uint32_t off = (lane & 31) + 2 * (lane >> 5);   // first base of the lane's window, 0..33
@@ -275,7 +295,8 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
             const uint32_t cm2 = MASK ? mk[t + 2] : 0u;
             bf16_mfma_tile<NCH, NB, MASK>(acc, onehot, bh, bl, funnel64(c0, c1, sh, sh_inv),
                                           MASK ? funnel32(cm0, cm1, off & 31) : 0u);
-            bf16_epilogue<NB>(a, st, acc, th, t * kBf16Tile, blk0, lane);
+            mfma_result_guard();
+            bf16_epilogue<NB>(a, st, acc, th, t * kBf16Tile, blk0, rtag, lane);
             c0 = c1; c1 = c2;
             if (MASK) { cm0 = cm1; cm1 = cm2; }
         }
@@ -302,21 +323,25 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
         if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
         bf16_mfma_tile<NCH, NB, MASK>(accB, onehot, bh, bl, funnel64(w0, w1, sh, sh_inv),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
-        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
+        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, rtag, lane);
         bf16_mfma_tile<NCH, NB, MASK>(accA, onehot, bh, bl, funnel64(w1, w2, sh, sh_inv),
                                       MASK ? funnel32(m1, m2, off & 31) : 0u);
-        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
+        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, rtag, lane);
         w0 = w2; w1 = w3; w2 = w4;
         if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
     if (t < t1) {                   // one tile left: t
         bf16_mfma_tile<NCH, NB, MASK>(accB, onehot, bh, bl, funnel64(w0, w1, sh, sh_inv),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
-        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
+        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, rtag, lane);
         pendingA = false;
-        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
+        mfma_result_guard();
+        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, rtag, lane);
     }
-    if (pendingA) bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
+    if (pendingA) {
+        mfma_result_guard();
+        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, rtag, lane);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -329,7 +354,7 @@ template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, const float (&bw)[NB][4 * NCH],
                                              const float (&th)[NB], uint32_t q0,
                                              uint64_t bits, uint32_t mb, uint32_t k, uint32_t blk0,
-                                             uint32_t lane)
+                                             uint32_t rtag, uint32_t lane)
 {
     constexpr int NP = 4 * NCH;
     f32x4 acc[NB];
@@ -346,16 +371,18 @@ __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, con
             acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bw[b][p], acc[b], 0, 0, 0);
     }
     const uint32_t qlane = q0 + 4 * k;     // windows qlane + 0..3
+    mfma_result_guard();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const float mx = fmaxf(fmaxf(acc[b][0], acc[b][1]), fmaxf(acc[b][2], acc[b][3]));
-        emit_lane(a, st, mx >= th[b], qlane, (blk0 + b) * kFp32Cols + (lane & 15));
+        emit_lane(a, st, mx >= th[b], qlane, ((blk0 + b) * kFp32Cols + (lane & 15)) | rtag);
     }
 }
 
 // Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1 (FP32 path).
 template <int NCH, int NB, bool MASK>
-__device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint32_t blk0, uint32_t t0, uint32_t t1,
+__device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint32_t blk0,
+                                           const float* __restrict__ thr, uint32_t rtag, uint32_t t0, uint32_t t1,
                                            uint32_t lane)
 {
     const float* __restrict__ fw = (const float*)a.frag_a;
@@ -366,7 +393,7 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
     for (int b = 0; b < NB; ++b) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) bw[b][p] = fw[((size_t)(blk0 + b) * kMaxLen + p) * 64 + lane];
-        th[b] = a.thr_lo[(blk0 + b) * kFp32Cols + (lane & 15)];
+        th[b] = thr[(blk0 + b) * kFp32Cols + (lane & 15)];
     }
     const uint32_t i = lane & 15, k = lane >> 4;
     const uint32_t s0 = 2 * i, s1 = 2 * (i + 16);    // bit offsets of the two sub-tiles
@@ -397,8 +424,8 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
                 const uint64_t b1 = funnel64(w[u], w[u + 1], s1, 63 - s1);
                 const uint32_t m0 = MASK ? funnel32(mw[u], mw[u + 1], i) : 0u;
                 const uint32_t m1 = MASK ? funnel32(mw[u], mw[u + 1], i + 16) : 0u;
-                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0, b0, m0, k, blk0, lane);
-                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0 + kFp32Tile, b1, m1, k, blk0, lane);
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0, b0, m0, k, blk0, rtag, lane);
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0 + kFp32Tile, b1, m1, k, blk0, rtag, lane);
             }
         }
         w[0] = w[kGroup];
@@ -411,29 +438,81 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
     }
 }
 
-// Work queue of the persistent filter.  The tile chunks are dealt to kXcds
-// sub-queues (chunk k to sub-queue k % kXcds); a workgroup starts on the
-// sub-queue of its XCD (workgroups are dispatched round-robin, so block b
-// runs on XCD b % kXcds) and steals from the others when it drains.  Inside a
-// sub-queue items run chunk-major: all column groups of a chunk follow each
-// other, so the chunk's packed words are fetched into that XCD's L2 once and
-// reused by every group.  Groups are numbered longest motifs first.
-constexpr int kMaxClasses = 8;
+// Work of the persistent filter is a set of items (tile chunk, column group).
+// Chunks are dealt to kXcds sub-queues (chunk k to sub-queue k % kXcds); a
+// workgroup starts on the sub-queue of its XCD (workgroups are dispatched
+// round-robin, so block b runs on XCD b % kXcds) and steals from the others
+// when it drains.  Inside a sub-queue items run chunk-major: all column groups
+// of a chunk follow each other, so the chunk's packed words are fetched into
+// that XCD's L2 once and reused by every group.  Groups are numbered longest
+// motifs first.
+//   implicit form (one stream, whole library): item j of sub-queue x is chunk
+//     (j / ngroups) * kXcds + x, group j % ngroups;
+//   list form (a batch of requests): sub-queue x holds
+//     items[sub_begin[x] .. sub_begin[x+1]), each (chunk, group), and a chunk
+//     is (first tile, end tile, request).
 constexpr int kXcds = 8;
 struct WorkQueue {
-    uint32_t nclasses;
+    uint32_t ntiles;
     uint32_t nchunks;
     uint32_t chunk_tiles;
-    uint32_t ngroups;                         // column groups of all classes
-    uint32_t group_base[kMaxClasses + 1];     // first group of each class
-    uint32_t nch[kMaxClasses];
-    uint32_t blk_begin[kMaxClasses];
+    uint32_t ngroups;
 };
 
-// One launch for the whole library: every resident wave pulls items until
+// Batch descriptor in device memory (list form).  The host writes it before
+// each launch, so one captured graph serves every batch that fits the
+// buffers it was captured with.
+struct BatchDesc {
+    uint32_t sub_begin[kXcds + 1];
+    uint32_t nreq;
+    uint32_t nbases;
+    uint32_t nruns;
+    uint32_t pad[4];
+};
+
+struct ListArgs {
+    const BatchDesc* desc;
+    const uint2* items;     // (chunk, group)
+    const uint4* chunks;    // (t0, t1, request, 0)
+};
+
+// One item: tiles [t0, t1) against one column group.
+template <bool BF, bool MASK>
+__device__ __forceinline__ void filter_item(const FilterArgs& a, Stage& st, const uint4* onehot, uint32_t grp,
+                                            const float* thr, uint32_t rtag, uint32_t t0, uint32_t t1,
+                                            uint32_t lane)
+{
+    const uint32_t blk0 = grp & 0xFFFFFu, nch = (grp >> 20) & 63u, nb = grp >> 26;
+    if constexpr (BF) {
+        switch (nch * 8 + nb) {
+#define BLAMM_BF(N, B) case N * 8 + B: bf16_range<N, B, MASK>(a, st, onehot, blk0, thr, rtag, t0, t1, lane); break;
+        BLAMM_BF(1, 1) BLAMM_BF(2, 1) BLAMM_BF(3, 1) BLAMM_BF(4, 1)
+        BLAMM_BF(5, 1) BLAMM_BF(6, 1) BLAMM_BF(7, 1) BLAMM_BF(8, 1)
+        BLAMM_BF(1, 2) BLAMM_BF(2, 2) BLAMM_BF(3, 2) BLAMM_BF(4, 2)
+        BLAMM_BF(5, 2) BLAMM_BF(6, 2) BLAMM_BF(7, 2) BLAMM_BF(8, 2)
+#undef BLAMM_BF
+        default: break;
+        }
+    } else {
+        (void)onehot;
+        switch (nch * 8 + nb) {
+#define BLAMM_F32(N, B) case N * 8 + B: fp32_range<N, B, MASK>(a, st, blk0, thr, rtag, t0, t1, lane); break;
+        BLAMM_F32(1, 1) BLAMM_F32(2, 1) BLAMM_F32(3, 1) BLAMM_F32(4, 1)
+        BLAMM_F32(5, 1) BLAMM_F32(6, 1) BLAMM_F32(7, 1) BLAMM_F32(8, 1)
+        BLAMM_F32(1, 2) BLAMM_F32(2, 2) BLAMM_F32(3, 2) BLAMM_F32(4, 2)
+        BLAMM_F32(5, 2) BLAMM_F32(6, 2) BLAMM_F32(7, 2) BLAMM_F32(8, 2)
+        BLAMM_F32(1, 4) BLAMM_F32(2, 4) BLAMM_F32(3, 4) BLAMM_F32(4, 4)
+        BLAMM_F32(5, 4) BLAMM_F32(6, 4) BLAMM_F32(7, 4) BLAMM_F32(8, 4)
+#undef BLAMM_F32
+        default: break;
+        }
+    }
+}
+
+// One launch for the whole work set: every resident wave pulls items until
 // all sub-queues are empty (one atomic per item).
-template <bool BF, int NB, bool MASK>
-__global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs a, WorkQueue q)
+template <bool BF, bool MASK, bool LIST>
+__global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs a, WorkQueue q, ListArgs l)
 {
     __shared__ uint4 onehot[MASK ? 64 : 16];
     if (BF && threadIdx.x < (MASK ? 64u : 16u)) onehot[threadIdx.x] = onehot_entry(threadIdx.x);
@@ -444,31 +523,37 @@ __global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs 
     const uint32_t home = blockIdx.x % kXcds;
     for (uint32_t hop = 0; hop < (uint32_t)kXcds; ++hop) {
         const uint32_t x = (home + hop) % kXcds;
-        if (x >= q.nchunks) continue;
-        const uint32_t items = ((q.nchunks - x + kXcds - 1) / kXcds) * q.ngroups;
+        uint32_t items = 0, base = 0;
+        if constexpr (LIST) {
+            base = __builtin_amdgcn_readfirstlane(l.desc->sub_begin[x]);
+            items = __builtin_amdgcn_readfirstlane(l.desc->sub_begin[x + 1]) - base;
+        } else {
+            if (x >= q.nchunks) continue;
+            items = ((q.nchunks - x + kXcds - 1) / kXcds) * q.ngroups;
+        }
         for (;;) {
             uint32_t j = 0;
             if (lane == 0) j = atomicAdd((unsigned int*)&a.counters[kCntQueue + x], 1u);
             j = __builtin_amdgcn_readfirstlane(__shfl(j, 0));
             if (j >= items) break;
-            const uint32_t k = (j / q.ngroups) * kXcds + x;
-            const uint32_t gi = j % q.ngroups;
-            uint32_t c = 0;
-            while (c + 1 < q.nclasses && gi >= q.group_base[c + 1]) ++c;
-            const uint32_t blk0 = q.blk_begin[c] + (gi - q.group_base[c]) * NB;
-            const uint32_t t0 = k * q.chunk_tiles;
-            const uint32_t t1 = min(a.ntiles, t0 + q.chunk_tiles);
-            switch (q.nch[c]) {
-#define BLAMM_CASE(N)                                                              \
-            case N:                                                                \
-                if constexpr (BF) bf16_range<N, NB, MASK>(a, st, onehot, blk0, t0, t1, lane); \
-                else fp32_range<N, NB, MASK>(a, st, blk0, t0, t1, lane);           \
-                break;
-            BLAMM_CASE(1) BLAMM_CASE(2) BLAMM_CASE(3) BLAMM_CASE(4)
-            BLAMM_CASE(5) BLAMM_CASE(6) BLAMM_CASE(7) BLAMM_CASE(8)
-#undef BLAMM_CASE
-            default: break;
+            uint32_t grp, t0, t1, rtag = 0;
+            const float* thr = a.thr_lo;
+            if constexpr (LIST) {
+                const uint2 it = l.items[base + j];
+                const uint4 ch = l.chunks[__builtin_amdgcn_readfirstlane(it.x)];
+                grp = __builtin_amdgcn_readfirstlane(a.groups[__builtin_amdgcn_readfirstlane(it.y)]);
+                t0 = __builtin_amdgcn_readfirstlane(ch.x);
+                t1 = __builtin_amdgcn_readfirstlane(ch.y);
+                const uint32_t req = __builtin_amdgcn_readfirstlane(ch.z);
+                rtag = req << 16;
+                thr = a.thr_lo + (size_t)req * a.req_stride;
+            } else {
+                const uint32_t k = (j / q.ngroups) * kXcds + x;
+                grp = __builtin_amdgcn_readfirstlane(a.groups[j % q.ngroups]);
+                t0 = k * q.chunk_tiles;
+                t1 = min(q.ntiles, t0 + q.chunk_tiles);
             }
+            filter_item<BF, MASK>(a, st, onehot, grp, thr, rtag, t0, t1, lane);
         }
     }
     stage_flush(a, st);
@@ -492,6 +577,9 @@ struct FinalizeArgs {
     const uint32_t* slot_col;             // caller's column index
     void* hits;                           // blamm_hit[]
     uint64_t hit_cap;
+    uint32_t req_stride;                  // list form: floats between two requests' tables
+    const BatchDesc* desc;                // list form: nbases, nruns from here
+    const uint32_t* req_end;              // list form: end of each request's bases
 };
 
 struct DevHit {
@@ -515,9 +603,17 @@ __device__ __forceinline__ uint32_t record_offset(int r)
 // proven band of the threshold is rescored in FP64 (the only FP64 step).
 // One thread per record; its NR windows share the column's weights and a
 // 128-bit window of packed bases that is shifted one position per step.
-template <int NR>
+// List form: records carry slot | request << 16, thresholds are per request,
+// a window must also end inside its request, and hits report
+// pos = request << 32 | batch position.
+template <int NR, bool LIST>
 __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
 {
+    uint32_t nbases = a.nbases, nruns = a.nruns;
+    if constexpr (LIST) {
+        nbases = __builtin_amdgcn_readfirstlane(a.desc->nbases);
+        nruns = __builtin_amdgcn_readfirstlane(a.desc->nruns);
+    }
     const uint64_t ncand_raw = a.counters[kCntCand];
     const uint64_t ncand = ncand_raw < a.cand_cap ? ncand_raw : a.cand_cap;
     const uint32_t lane = threadIdx.x & 63;
@@ -537,16 +633,21 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             if ((uint64_t)at0 + i < a.hit_cap) hits[at0 + i] = sbuf[i];
         sn = 0;
     };
-    constexpr uint32_t span = NR == 16 ? 28 : 4;    // windows covered by a record
     for (uint64_t base = wave * 64; base < ncand; base += nw * 64) {
         const uint64_t idx = base + lane;
         const bool have = idx < ncand;
-        uint32_t q0 = 0, slot = 0;
+        uint32_t q0 = 0, slot = 0, req = 0, qend = nbases;
         if (have) {
             const uint2 cd = a.cand[idx];
             q0 = cd.x;
             slot = cd.y;
+            if constexpr (LIST) {
+                slot = cd.y & 0xFFFFu;
+                req = cd.y >> 16;
+                qend = min(nbases, a.req_end[req]);
+            }
         }
+        const size_t toff = LIST ? (size_t)req * a.req_stride + slot : (size_t)slot;
         const int L = have ? a.slot_len[slot] : 0;
         const float* w = a.slot_w + (size_t)slot * kMaxLen * 4;
         // packed bases q0 .. q0+63 and their lowercase bits
@@ -556,12 +657,12 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             lo = load_bits64(a.packed, 2ull * q0);
             hi = load_bits64(a.packed, 2ull * q0 + 64);
             if (a.lcmask) mlo = load_bits64(a.lcmask, q0);
-            uint32_t l = 0, h = a.nruns;
+            uint32_t l = 0, h = nruns;
             while (l < h) {
                 const uint32_t mid = (l + h) >> 1;
                 if (a.run_starts[mid] <= q0) l = mid + 1; else h = mid;
             }
-            nb1 = l < a.nruns ? (uint32_t)a.run_starts[l] : a.nbases;
+            nb1 = l < nruns ? (uint32_t)a.run_starts[l] : nbases;
         }
         float s[NR];
 #pragma unroll
@@ -580,32 +681,31 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             wh >>= 2;
             ml >>= 1;
         }
-        const float thr = have ? a.slot_thr[slot] : 0.f;
-        const float shi = have ? a.slot_seq_hi[slot] : 0.f;
-        const float slo = have ? a.slot_seq_lo[slot] : 0.f;
+        const float thr = have ? a.slot_thr[toff] : 0.f;
+        const float shi = have ? a.slot_seq_hi[toff] : 0.f;
+        const float slo = have ? a.slot_seq_lo[toff] : 0.f;
         const uint32_t col = have ? a.slot_col[slot] : 0u;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             const uint32_t q = q0 + record_offset<NR>(r);
             bool hit = false;
             float score = s[r];
-            if (have && q < a.nbases && s[r] >= slo) {
+            if (have && q < qend && s[r] >= slo) {
                 // run-boundary rule: next run start after q
                 uint32_t nb = nb1;
                 if (q >= nb1) {
-                    uint32_t l = 0, h = a.nruns;
+                    uint32_t l = 0, h = nruns;
                     while (l < h) {
                         const uint32_t mid = (l + h) >> 1;
                         if (a.run_starts[mid] <= q) l = mid + 1; else h = mid;
                     }
-                    nb = l < a.nruns ? (uint32_t)a.run_starts[l] : a.nbases;
+                    nb = l < nruns ? (uint32_t)a.run_starts[l] : nbases;
                 }
-                if ((uint64_t)q + (uint64_t)L <= (uint64_t)nb) {
+                if ((uint64_t)q + (uint64_t)L <= (uint64_t)min(nb, qend)) {
                     if (s[r] >= shi) {
                         hit = true;
                     } else {
                         // within the FP32 band: FP64 left-to-right sum
-                        const uint32_t o = record_offset<NR>(r);
                         const uint64_t bits = load_bits64(a.packed, 2ull * q);
                         const uint64_t mb = a.lcmask ? load_bits64(a.lcmask, q) : 0ull;
                         double d = 0.0;
@@ -613,7 +713,6 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
                             if ((mb >> p) & 1) continue;
                             d += (double)w[4 * p + ((bits >> (2 * p)) & 3)];
                         }
-                        (void)o;
                         atomicAdd(&a.counters_rw[kCntBand], 1ull);
                         if (d >= (double)thr) {
                             hit = true;
@@ -629,7 +728,7 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 if (hit) {
                     DevHit hh;
-                    hh.pos = q;
+                    hh.pos = LIST ? ((uint64_t)req << 32 | q) : (uint64_t)q;
                     hh.col = col;
                     hh.score = score;
                     sbuf[sn + rank] = hh;
@@ -637,7 +736,6 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
                 sn += (uint32_t)__popcll(m);
             }
         }
-        (void)span;
     }
     flush();
 }

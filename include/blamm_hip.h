@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define BLAMM_HIP_ABI_VERSION 1
+#define BLAMM_HIP_ABI_VERSION 2
 
 /* Longest motif the engine scores (positions); longer motifs are rejected by
  * blamm_hip_set_library with BLAMM_ERR_ARG. */
@@ -149,6 +149,130 @@ int blamm_hip_scan_finish(blamm_hip_ctx *ctx, uint64_t *nhits);
 
 /* Counters of the last finished scan. */
 int blamm_hip_last_stats(blamm_hip_ctx *ctx, blamm_scan_stats *stats);
+
+/* ------------------------------------------------------------------------
+ * Tile shapes of the filter.  The reference picks column tiles of its
+ * length-sorted pattern matrix by a recursive split that trades zero rows
+ * against the number of sgemm calls (generateMatrixTiles / findBestSplit,
+ * motif.cpp:451-540); here the same trade-off is three knobs of the MFMA
+ * filter, set directly or by blamm_hip_autotune().
+ * ------------------------------------------------------------------------ */
+typedef struct blamm_tuning {
+    uint32_t chunk_tiles;      /* window tile: 32-window tiles per work item */
+    uint32_t blocks_per_wave;  /* motif tile: column blocks (32 bf16 / 16 fp32
+                                  columns each) a wave scores at once: 1, 2
+                                  (bf16 at most 2, fp32 at most 4) */
+    uint32_t kpad_positions;   /* K-padding: a block's motif length is rounded
+                                  up to a multiple of 4 or 8 positions */
+} blamm_tuning;
+
+int blamm_hip_get_tuning(blamm_hip_ctx *ctx, blamm_tuning *t);
+/* Applies to later scans; re-lays the resident library out when the motif
+ * tile or the K-padding changes. */
+int blamm_hip_set_tuning(blamm_hip_ctx *ctx, const blamm_tuning *t);
+
+/* Time full scans of a device-resident sample under candidate tunings and
+ * keep the fastest.  cache_path (may be NULL): a text file of earlier
+ * results, keyed by device, precision, library length histogram and sample
+ * size; a matching entry is applied without timing, a new result is
+ * appended.  *chosen / *chosen_ms (may be NULL) receive the setting and its
+ * scan time. */
+int blamm_hip_autotune(blamm_hip_ctx *ctx, const uint64_t *d_packed, uint64_t packed_words,
+                       const uint64_t *d_lcmask, uint64_t mask_words, uint64_t nbases,
+                       const uint64_t *d_run_starts, uint64_t nruns, const char *cache_path,
+                       blamm_tuning *chosen, double *chosen_ms);
+
+/* ------------------------------------------------------------------------
+ * Batches: many requests in one persistent launch.  The reference scans one
+ * manifest group's sequences as overlapping blocks handed to worker threads
+ * (FastaBatch::getNextOverlappingBlock, sequence.cpp:274-293; scanPWMCUBLAS,
+ * pwmscan.cpp:439-453); a batch instead packs several independent streams,
+ * each with its own thresholds, into one work list scored by one launch
+ * sequence that is captured in a hipGraph and replayed.
+ * ------------------------------------------------------------------------ */
+typedef struct blamm_request {
+    const uint64_t *packed;      /* host, (nbases+31)/32 words, layout as above */
+    const uint64_t *lcmask;      /* host, (nbases+63)/64 words, or NULL */
+    uint64_t nbases;
+    const uint64_t *run_starts;  /* sorted run starts of this stream */
+    uint64_t nruns;
+    const float *thresholds;     /* ncols thresholds in library column order;
+                                    +INFINITY leaves a column out; NULL = the
+                                    library's thresholds */
+    uint64_t expected_hits;      /* admission estimate for the server; 0 = 1e-4
+                                    per base and requested column */
+} blamm_request;
+
+typedef struct blamm_hip_batch blamm_hip_batch;
+
+/* A batch slot on ctx's device with its own stream; device buffers start
+ * sized for max_bases / max_requests and grow on demand.  It uses the
+ * context's library and tuning: do not change them while a batch is in
+ * flight. */
+int blamm_hip_batch_create(blamm_hip_ctx *ctx, uint64_t max_bases, uint32_t max_requests,
+                           blamm_hip_batch **batch);
+int blamm_hip_batch_destroy(blamm_hip_batch *batch);
+/* Device bytes the slot holds now. */
+uint64_t blamm_hip_batch_device_bytes(const blamm_hip_batch *batch);
+/* Stage nreq (< 65536) requests and launch them (asynchronous).  Request
+ * buffers may be reused as soon as this returns. */
+int blamm_hip_batch_submit(blamm_hip_batch *batch, const blamm_request *reqs, uint32_t nreq);
+/* 0 when the batch has finished on the device (or none is in flight),
+ * 1 while it runs. */
+int blamm_hip_batch_query(blamm_hip_batch *batch);
+/* Wait for the batch (replaying it after growing scratch or the hit buffer
+ * when either overflowed); *nhits = hits of all requests. */
+int blamm_hip_batch_wait(blamm_hip_batch *batch, uint64_t *nhits);
+/* Hits of request `req` of the last finished batch, positions local to that
+ * request's stream; valid until the next submit. */
+int blamm_hip_batch_hits(blamm_hip_batch *batch, uint32_t req, const blamm_hit **hits, uint64_t *n);
+/* kernel_ms = the whole captured launch sequence of the last batch. */
+int blamm_hip_batch_stats(blamm_hip_batch *batch, blamm_scan_stats *stats);
+
+/* ------------------------------------------------------------------------
+ * Server: continuous batching over a resident library.  No counterpart in
+ * the reference (its scan is a one-shot command); built on the batch slots.
+ * ------------------------------------------------------------------------ */
+typedef struct blamm_server_config {
+    uint64_t hbm_budget_bytes;    /* device bytes for batches in flight (0 = 8 GiB),
+                                     split evenly over the slots */
+    uint32_t slots;               /* batches in flight, 1..4 (0 = 2) */
+    uint32_t max_batch_requests;  /* requests per batch (0 = 256) */
+} blamm_server_config;
+
+typedef struct blamm_request_info {
+    int status;                   /* BLAMM_OK or the batch's error */
+    uint32_t batch_requests;      /* requests in the batch it ran in */
+    uint64_t nhits;
+    uint64_t t_submit_ns, t_start_ns, t_done_ns;   /* steady clock */
+} blamm_request_info;
+
+typedef struct blamm_server_stats {
+    uint64_t submitted, completed, batches, hits;
+    double device_ms;             /* summed batch launch times */
+} blamm_server_stats;
+
+typedef struct blamm_hip_server blamm_hip_server;
+
+/* Starts a worker thread that owns the batch slots.  While the server runs,
+ * the context's library and tuning must not change. */
+int blamm_hip_server_create(blamm_hip_ctx *ctx, const blamm_server_config *cfg,
+                            blamm_hip_server **server);
+/* Queue a request.  Its packed / lcmask / run_starts buffers must stay valid
+ * until blamm_hip_server_wait() returns for it; thresholds are copied.
+ * Requests larger than one slot's budget are rejected with BLAMM_ERR_ARG. */
+int blamm_hip_server_submit(blamm_hip_server *server, const blamm_request *req, uint64_t *id);
+/* BLAMM_OK when request id has finished, 1 on timeout (timeout_ms < 0 waits
+ * without limit). */
+int blamm_hip_server_wait(blamm_hip_server *server, uint64_t id, int64_t timeout_ms);
+/* Copy the finished request's hits (request-local positions) and timing,
+ * then forget the request.  BLAMM_ERR_OVERFLOW (request kept) when cap is
+ * below info->nhits. */
+int blamm_hip_server_result(blamm_hip_server *server, uint64_t id, blamm_hit *hits, uint64_t cap,
+                            blamm_request_info *info);
+int blamm_hip_server_stats(blamm_hip_server *server, blamm_server_stats *stats);
+/* Finishes queued and running requests, then stops the worker. */
+int blamm_hip_server_destroy(blamm_hip_server *server);
 
 #ifdef __cplusplus
 }

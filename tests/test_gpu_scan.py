@@ -112,6 +112,35 @@ def test_many_classes_and_partial_blocks(scanner, orc, prec):
     check_against_oracle(scanner, orc, text, runs, pwms, thr, prec)
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_repeated_scans_are_identical(scanner, orc, prec):
+    """The same scan repeated under every tile shape gives the oracle's hit
+    set each time.  Guards against accumulator reads that race the matrix
+    cores (seen once as about one wrong FP32 scan in ten)."""
+    text, runs, bg = stream_case(5, 120_000, lower_every=0)
+    ms = synth.motifs(40, 1, 8, 0.5, seed=5)
+    pwms, thr = library(ms, bg, pvalue=1e-3)
+    P, lengths, _ = ba.build_pattern_matrix(pwms)
+    scanner.set_library(P, lengths, thr, prec)
+    want = hit_set(orc.scan_fp64(text, runs, pwms, thr))
+    packed, lower = ba.pack(text)
+    nbs = [1, 2] if prec == ba.PREC_BF16X2 else [1, 2, 4]
+    bad = []
+    try:
+        for rep in range(2):
+            for nb in nbs:
+                for kp in (4, 8):
+                    for ct in (1, 512):
+                        scanner.set_tuning(ct, nb, kp)
+                        got = hit_set(scanner.scan(packed, len(text), runs, lower))
+                        if got != want:
+                            bad.append((rep, nb, kp, ct, len(want - got), len(got - want)))
+    finally:
+        scanner.set_tuning(512, 2, 4)
+    say(f"repeat prec={prec}: {len(bad)} wrong")
+    assert not bad, bad
+
+
 def test_hit_capacity_overflow_is_reported(scanner):
     text, runs, bg = stream_case(61, 30_000)
     ms = synth.motifs(8, 5, 12, 0.5, seed=61)
