@@ -151,6 +151,7 @@ def hip_lib() -> ctypes.CDLL:
         lib.blamm_hip_server_result.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_uint64,
                                                 ctypes.POINTER(RequestInfo)]
         lib.blamm_hip_server_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        lib.blamm_hip_server_release.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(RequestInfo)]
         lib.blamm_hip_server_destroy.argtypes = [vp]
         _hip = lib
     return _hip
@@ -468,6 +469,14 @@ class Server:
         self._keep.pop(rid, None)
         _check(rc)
         return hits, info
+
+    def release(self, rid: int) -> RequestInfo:
+        """Timing and hit count of a finished request; its hits are dropped."""
+        info = RequestInfo()
+        rc = self.lib.blamm_hip_server_release(self.s, rid, ctypes.byref(info))
+        self._keep.pop(rid, None)
+        _check(rc)
+        return info
 
     def stats(self) -> ServerStats:
         st = ServerStats()

@@ -8,6 +8,7 @@
 // Included once, at the end of blamm_hip.hip (same translation unit as the
 // kernels and the library layout).
 
+#include <memory>
 #include <mutex>
 
 namespace {
@@ -16,6 +17,56 @@ constexpr uint64_t kReqAlign = 64;          // bases; keeps both packed and mask
 constexpr uint64_t kBatchMaxBases = (1ull << 32) - 4096;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Pinned host buffers for finished batches' hits.  A buffer stays with the
+// requests that point into it (the server hands out slices without copying)
+// and returns to the pool when the last of them lets go, so a busy server
+// recycles pinned memory instead of allocating and freeing it.
+struct HitBuf {
+    blamm_hit* p = nullptr;
+    size_t n = 0;
+};
+
+struct HitPool {
+    std::mutex mu;
+    std::vector<HitBuf> free;
+    ~HitPool()
+    {
+        for (HitBuf& h : free) (void)hipHostFree(h.p);
+    }
+};
+
+int acquire_hits(const std::shared_ptr<HitPool>& pool, size_t want, std::shared_ptr<HitBuf>* out)
+{
+    HitBuf got;
+    {
+        std::lock_guard<std::mutex> g(pool->mu);
+        size_t best = pool->free.size();
+        for (size_t i = 0; i < pool->free.size(); ++i)
+            if (pool->free[i].n >= want && (best == pool->free.size() || pool->free[i].n < pool->free[best].n))
+                best = i;
+        if (best < pool->free.size()) {
+            got = pool->free[best];
+            pool->free.erase(pool->free.begin() + best);
+        }
+    }
+    if (!got.p) {
+        got.n = std::max<size_t>(want + want / 4, 1u << 16);
+        hipError_t e = hipHostMalloc((void**)&got.p, got.n * sizeof(blamm_hit), hipHostMallocDefault);
+        if (e != hipSuccess)
+            return fail(e == hipErrorOutOfMemory ? BLAMM_ERR_NOMEM : BLAMM_ERR_DEVICE,
+                        std::string("pinned hit buffer: ") + hipGetErrorString(e));
+    }
+    std::shared_ptr<HitPool> keep = pool;
+    *out = std::shared_ptr<HitBuf>(new HitBuf(got), [keep](HitBuf* h) {
+        {
+            std::lock_guard<std::mutex> g(keep->mu);
+            keep->free.push_back(*h);
+        }
+        delete h;
+    });
+    return BLAMM_OK;
+}
 
 }  // namespace
 
@@ -26,19 +77,21 @@ struct blamm_hip_batch {
 
     // device buffers
     DevBuf<uint64_t> packed, lcmask, runs;
-    DevBuf<float> tab;              // [nreq][4][nslots]: filter thr, thr, seq lo, seq hi
-    DevBuf<uint32_t> req_end;
+    DevBuf<float> tab;              // [nreq][nslots] thresholds
+    DevBuf<uint32_t> req_start, req_end, req_count;
+    DevBuf<unsigned long long> req_off, cursor;
     DevBuf<uint2> items, cand;
     DevBuf<uint4> chunks;
     DevBuf<BatchDesc> desc;
     DevBuf<unsigned long long> counters;
-    DevBuf<blamm_hit> hits;
+    DevBuf<blamm_hit> hits, grouped;
     uint64_t buf_gen = 1;           // bumped whenever a device buffer moves
 
     // pinned staging
     HostBuf<uint8_t> stage;
-    HostBuf<unsigned long long> cnt_h;
-    HostBuf<blamm_hit> hits_h;
+    HostBuf<unsigned long long> cnt_h, off_h;
+    std::shared_ptr<HitPool> pool = std::make_shared<HitPool>();
+    std::shared_ptr<HitBuf> out;    // hits of the last finished batch, grouped by request
 
     // captured launch sequence per lowercase variant
     hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -49,8 +102,7 @@ struct blamm_hip_batch {
     bool mask = false;
     bool done = false;
     uint32_t nreq = 0;
-    std::vector<uint64_t> req_start, req_n;
-    std::vector<blamm_hit> out;
+    std::vector<uint64_t> start_h, n_h;
     std::vector<uint64_t> out_off;
     blamm_scan_stats stats{};
 };
@@ -80,7 +132,6 @@ int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
 {
     blamm_hip_ctx* c = b->ctx;
     const bool bf = c->precision == BLAMM_PREC_BF16X2;
-    const size_t ns = (size_t)c->nslots;
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, b->stream, b->counters.p);
     HIP_TRY(hipGetLastError());
     const FilterArgs a = filter_args(c, b->packed.p, mask ? b->lcmask.p : nullptr, b->tab.p, b->cand.p, b->cand.n,
@@ -91,15 +142,31 @@ int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
     if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch filter launch: ") + hipGetErrorString(e));
     FinalizeArgs f = finalize_args(c, b->cand.p, b->cand.n, b->counters.p, b->packed.p, mask ? b->lcmask.p : nullptr,
                                    b->runs.p);
-    f.slot_thr = b->tab.p + ns;
-    f.slot_seq_lo = b->tab.p + 2 * ns;
-    f.slot_seq_hi = b->tab.p + 3 * ns;
+    f.slot_thr = b->tab.p;
     f.hits = b->hits.p;
     f.hit_cap = b->hits.n;
     f.desc = b->desc.p;
     f.req_end = b->req_end.p;
     e = launch_finalize(bf, true, f, (uint32_t)c->num_cu * 8, b->stream);
     if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch finalize launch: ") + hipGetErrorString(e));
+    // group the hits by request on the device
+    GroupArgs g;
+    g.desc = b->desc.p;
+    g.counters = b->counters.p;
+    g.cap = std::min(b->hits.n, b->grouped.n);
+    g.in = (const DevHit*)b->hits.p;
+    g.out = (DevHit*)b->grouped.p;
+    g.req_count = b->req_count.p;
+    g.req_off = b->req_off.p;
+    g.cursor = b->cursor.p;
+    g.req_start = b->req_start.p;
+    const uint32_t gg = (uint32_t)c->num_cu * 4;
+    hipLaunchKernelGGL(group_clear_kernel, dim3(64), dim3(kGroupThreads), 0, b->stream, g);
+    hipLaunchKernelGGL(group_count_kernel, dim3(gg), dim3(kGroupThreads), 0, b->stream, g);
+    hipLaunchKernelGGL(group_scan_kernel, dim3(1), dim3(1024), 0, b->stream, g);
+    hipLaunchKernelGGL(group_scatter_kernel, dim3(gg), dim3(kGroupThreads), 0, b->stream, g);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch grouping launch: ") + hipGetErrorString(e));
     return BLAMM_OK;
 }
 
@@ -137,6 +204,8 @@ int batch_launch(blamm_hip_batch* b)
     HIP_TRY(hipEventRecord(b->ev_end, b->stream));
     HIP_TRY(hipMemcpyAsync(b->cnt_h.p, b->counters.p, kCntSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            b->stream));
+    HIP_TRY(hipMemcpyAsync(b->off_h.p, b->req_off.p, ((size_t)b->nreq + 1) * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipEventRecord(b->ev_done, b->stream));
     return BLAMM_OK;
 }
@@ -164,6 +233,7 @@ int blamm_hip_batch_create(blamm_hip_ctx* c, uint64_t max_bases, uint32_t max_re
     if (e == hipSuccess) e = b->packed.reserve(nb / 32 + nr * 2 + 16);
     if (e == hipSuccess) e = b->cand.reserve(std::max<uint64_t>(nb / 256, 1u << 16));
     if (e == hipSuccess) e = b->hits.reserve(std::max<uint64_t>(nb / 256, 1u << 16));
+    if (e == hipSuccess) e = b->grouped.reserve(b->hits.n);
     if (e != hipSuccess) {
         blamm_hip_batch_destroy(b);
         return fail(e == hipErrorOutOfMemory ? BLAMM_ERR_NOMEM : BLAMM_ERR_DEVICE,
@@ -179,10 +249,12 @@ int blamm_hip_batch_destroy(blamm_hip_batch* b)
     (void)hipSetDevice(b->ctx->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    b->packed.release(); b->lcmask.release(); b->runs.release(); b->tab.release(); b->req_end.release();
+    b->packed.release(); b->lcmask.release(); b->runs.release(); b->tab.release(); b->req_start.release();
+    b->req_end.release(); b->req_count.release(); b->req_off.release(); b->cursor.release();
     b->items.release(); b->cand.release(); b->chunks.release(); b->desc.release(); b->counters.release();
-    b->hits.release();
-    b->stage.release(); b->cnt_h.release(); b->hits_h.release();
+    b->hits.release(); b->grouped.release();
+    b->stage.release(); b->cnt_h.release(); b->off_h.release();
+    b->out.reset();
     if (b->ev_start) (void)hipEventDestroy(b->ev_start);
     if (b->ev_end) (void)hipEventDestroy(b->ev_end);
     if (b->ev_done) (void)hipEventDestroy(b->ev_done);
@@ -194,8 +266,9 @@ int blamm_hip_batch_destroy(blamm_hip_batch* b)
 uint64_t blamm_hip_batch_device_bytes(const blamm_hip_batch* b)
 {
     if (!b) return 0;
-    return b->packed.n * 8 + b->lcmask.n * 8 + b->runs.n * 8 + b->tab.n * 4 + b->req_end.n * 4 + b->items.n * 8 +
-           b->cand.n * 8 + b->chunks.n * 16 + b->hits.n * sizeof(blamm_hit) + sizeof(BatchDesc) +
+    return b->packed.n * 8 + b->lcmask.n * 8 + b->runs.n * 8 + b->tab.n * 4 +
+           (b->req_start.n + b->req_end.n + b->req_count.n) * 4 + (b->req_off.n + b->cursor.n) * 8 + b->items.n * 8 +
+           b->cand.n * 8 + b->chunks.n * 16 + (b->hits.n + b->grouped.n) * sizeof(blamm_hit) + sizeof(BatchDesc) +
            kCntSlots * 8;
 }
 
@@ -212,8 +285,8 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
 
     // Layout of the batch stream: request r occupies [start_r, start_r + n_r),
     // starts aligned to kReqAlign bases.
-    b->req_start.assign(nreq, 0);
-    b->req_n.assign(nreq, 0);
+    b->start_h.assign(nreq, 0);
+    b->n_h.assign(nreq, 0);
     bool mask = false;
     uint64_t total = 0, nruns = 0;
     for (uint32_t r = 0; r < nreq; ++r) {
@@ -227,8 +300,8 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
             for (int j = 0; j < c->ncols; ++j)
                 if (std::isnan(q.thresholds[j]))
                     return fail(BLAMM_ERR_ARG, "batch_submit: NaN threshold in request " + std::to_string(r));
-        b->req_start[r] = total;
-        b->req_n[r] = q.nbases;
+        b->start_h[r] = total;
+        b->n_h[r] = q.nbases;
         total = align_up(total + q.nbases, kReqAlign);
         if (total > kBatchMaxBases) return fail(BLAMM_ERR_ARG, "batch_submit: batch longer than 2^32 - 4096 bases");
         mask = mask || (q.lcmask != nullptr && q.nbases > 0);
@@ -239,23 +312,15 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     const int cpb = c->cols_per_blk;
     const uint32_t ct = std::max<uint32_t>(1, c->tuning.chunk_tiles);
 
-    // Active groups and thresholds per request.
+    // Thresholds in slot order and active column groups per request.
     std::vector<std::vector<uint32_t>> active(nreq);
-    std::vector<float> tabh(4 * (size_t)nreq * nslots, INFINITY);
+    std::vector<float> tabh((size_t)nreq * nslots, INFINITY);
     for (uint32_t r = 0; r < nreq; ++r) {
         const float* thr = reqs[r].thresholds ? reqs[r].thresholds : c->lib_thr.data();
-        float* t0 = &tabh[((size_t)r * 4 + 0) * nslots];
-        float* t1 = &tabh[((size_t)r * 4 + 1) * nslots];
-        float* t2 = &tabh[((size_t)r * 4 + 2) * nslots];
-        float* t3 = &tabh[((size_t)r * 4 + 3) * nslots];
+        float* t0 = &tabh[(size_t)r * nslots];
         for (size_t s = 0; s < nslots; ++s) {
             const int col = c->slot_col_h[s];
-            if (col < 0 || thr[col] == INFINITY) continue;
-            const SlotThr st = slot_thresholds(thr[col], c->band_h[s], c->seq_h[s]);
-            t0[s] = st.lo;
-            t1[s] = st.thr;
-            t2[s] = st.seq_lo;
-            t3[s] = st.seq_hi;
+            if (col >= 0) t0[s] = thr[col];
         }
         for (uint32_t g = 0; g < (uint32_t)c->groups.size(); ++g) {
             const GroupInfo& gi = c->groups[g];
@@ -269,9 +334,9 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     std::vector<uint4> chunks;
     std::vector<std::vector<uint2>> sub(kXcds);
     for (uint32_t r = 0; r < nreq; ++r) {
-        if (b->req_n[r] == 0 || active[r].empty()) continue;
-        const uint32_t tb = (uint32_t)(b->req_start[r] / kBf16Tile);
-        const uint32_t te = (uint32_t)((b->req_start[r] + b->req_n[r] + kBf16Tile - 1) / kBf16Tile);
+        if (b->n_h[r] == 0 || active[r].empty()) continue;
+        const uint32_t tb = (uint32_t)(b->start_h[r] / kBf16Tile);
+        const uint32_t te = (uint32_t)((b->start_h[r] + b->n_h[r] + kBf16Tile - 1) / kBf16Tile);
         for (uint32_t t = tb; t < te; t += ct) {
             const uint32_t k = (uint32_t)chunks.size();
             chunks.push_back(make_uint4(t, std::min(te, t + ct), r, 0));
@@ -295,7 +360,15 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     if (mask) BLAMM_TRY(batch_grow(b, b->lcmask, mw));
     BLAMM_TRY(batch_grow(b, b->runs, nruns + 1));
     BLAMM_TRY(batch_grow(b, b->tab, tabh.size()));
+    BLAMM_TRY(batch_grow(b, b->req_start, nreq));
     BLAMM_TRY(batch_grow(b, b->req_end, nreq));
+    BLAMM_TRY(batch_grow(b, b->req_count, nreq));
+    BLAMM_TRY(batch_grow(b, b->req_off, (size_t)nreq + 1));
+    BLAMM_TRY(batch_grow(b, b->cursor, nreq));
+    if (b->off_h.n < (size_t)nreq + 1) {
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        HIP_TRY(b->off_h.reserve((size_t)nreq + 1 + nreq / 8));
+    }
     BLAMM_TRY(batch_grow(b, b->items, std::max<size_t>(nitems, 1)));
     BLAMM_TRY(batch_grow(b, b->chunks, std::max<size_t>(chunks.size(), 1)));
 
@@ -305,7 +378,8 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     const size_t o_rn = align_up(o_mk + mw * 8, 256);
     const size_t o_tb = align_up(o_rn + (nruns + 1) * 8, 256);
     const size_t o_re = align_up(o_tb + tabh.size() * 4, 256);
-    const size_t o_ch = align_up(o_re + (size_t)nreq * 4, 256);
+    const size_t o_rs = align_up(o_re + (size_t)nreq * 4, 256);
+    const size_t o_ch = align_up(o_rs + (size_t)nreq * 4, 256);
     const size_t o_it = align_up(o_ch + chunks.size() * 16, 256);
     const size_t o_ds = align_up(o_it + nitems * 8, 256);
     const size_t o_end = o_ds + sizeof(BatchDesc);
@@ -318,12 +392,13 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     uint64_t* mk = (uint64_t*)(S + o_mk);
     uint64_t* rn = (uint64_t*)(S + o_rn);
     uint32_t* re = (uint32_t*)(S + o_re);
+    uint32_t* rs = (uint32_t*)(S + o_rs);
     std::memset(pk, 0, pw * 8);
     if (mask) std::memset(mk, 0, mw * 8);
     size_t ir = 0;
     for (uint32_t r = 0; r < nreq; ++r) {
         const blamm_request& q = reqs[r];
-        const uint64_t s0 = b->req_start[r];
+        const uint64_t s0 = b->start_h[r];
         if (q.nbases > 0) {
             std::memcpy(pk + s0 / 32, q.packed, ((q.nbases + 31) / 32) * 8);
             if (q.lcmask) std::memcpy(mk + s0 / 64, q.lcmask, ((q.nbases + 63) / 64) * 8);
@@ -332,6 +407,7 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
         for (uint64_t i = 0; i < q.nruns; ++i)
             if (q.run_starts[i] > 0 && q.run_starts[i] < q.nbases) rn[ir++] = s0 + q.run_starts[i];
         re[r] = (uint32_t)(s0 + q.nbases);
+        rs[r] = (uint32_t)s0;
     }
     d.nruns = (uint32_t)ir;
     std::memcpy(S + o_tb, tabh.data(), tabh.size() * 4);
@@ -349,6 +425,7 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     HIP_TRY(hipMemcpyAsync(b->runs.p, S + o_rn, std::max<size_t>(ir, 1) * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(b->tab.p, S + o_tb, tabh.size() * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(b->req_end.p, S + o_re, (size_t)nreq * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->req_start.p, S + o_rs, (size_t)nreq * 4, hipMemcpyHostToDevice, s));
     if (!chunks.empty()) HIP_TRY(hipMemcpyAsync(b->chunks.p, S + o_ch, chunks.size() * 16, hipMemcpyHostToDevice, s));
     if (nitems) HIP_TRY(hipMemcpyAsync(b->items.p, S + o_it, nitems * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(b->desc.p, S + o_ds, sizeof(BatchDesc), hipMemcpyHostToDevice, s));
@@ -394,7 +471,10 @@ int blamm_hip_batch_wait(blamm_hip_batch* b, uint64_t* nhits)
         // Grow what overflowed and replay the batch (its inputs are still on
         // the device).
         if (cand > b->cand.n) BLAMM_TRY(batch_grow(b, b->cand, cand + cand / 4 + 1024));
-        if (nh > b->hits.n) BLAMM_TRY(batch_grow(b, b->hits, nh + nh / 4 + 1024));
+        if (nh > b->hits.n) {
+            BLAMM_TRY(batch_grow(b, b->hits, nh + nh / 4 + 1024));
+            BLAMM_TRY(batch_grow(b, b->grouped, b->hits.n));
+        }
         BLAMM_TRY(batch_launch(b));
     }
     float ms = 0.f;
@@ -404,23 +484,15 @@ int blamm_hip_batch_wait(blamm_hip_batch* b, uint64_t* nhits)
     b->stats.hits = nh;
     b->stats.kernel_ms = ms;
     b->stats.filter_ms = 0.f;
-    if (nh > b->hits_h.n) HIP_TRY(b->hits_h.reserve(nh + nh / 8));
+    // hits arrive grouped by request with request-local positions
+    std::shared_ptr<HitBuf> buf;
+    BLAMM_TRY(acquire_hits(b->pool, std::max<uint64_t>(nh, 1), &buf));
     if (nh > 0) {
-        HIP_TRY(hipMemcpyAsync(b->hits_h.p, b->hits.p, nh * sizeof(blamm_hit), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipMemcpyAsync(buf->p, b->grouped.p, nh * sizeof(blamm_hit), hipMemcpyDeviceToHost, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
     }
-    // Group by request; positions become request-local.
-    b->out_off.assign((size_t)b->nreq + 1, 0);
-    for (uint64_t i = 0; i < nh; ++i) b->out_off[(b->hits_h.p[i].pos >> 32) + 1]++;
-    for (uint32_t r = 0; r < b->nreq; ++r) b->out_off[r + 1] += b->out_off[r];
-    b->out.resize(nh);
-    std::vector<uint64_t> fill(b->out_off.begin(), b->out_off.end() - 1);
-    for (uint64_t i = 0; i < nh; ++i) {
-        blamm_hit h = b->hits_h.p[i];
-        const uint32_t r = (uint32_t)(h.pos >> 32);
-        h.pos = (h.pos & 0xFFFFFFFFull) - b->req_start[r];
-        b->out[fill[r]++] = h;
-    }
+    b->out = buf;
+    b->out_off.assign(b->off_h.p, b->off_h.p + b->nreq + 1);
     b->pending = false;
     b->done = true;
     if (nhits) *nhits = nh;
@@ -432,7 +504,7 @@ int blamm_hip_batch_hits(blamm_hip_batch* b, uint32_t req, const blamm_hit** hit
     if (!b || !hits || !n) return fail(BLAMM_ERR_ARG, "batch_hits: null pointer");
     if (!b->done) return fail(BLAMM_ERR_STATE, "batch_hits: no finished batch");
     if (req >= b->nreq) return fail(BLAMM_ERR_ARG, "batch_hits: request index out of range");
-    *hits = b->out.data() + b->out_off[req];
+    *hits = b->out->p + b->out_off[req];
     *n = b->out_off[req + 1] - b->out_off[req];
     return BLAMM_OK;
 }

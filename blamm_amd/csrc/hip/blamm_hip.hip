@@ -110,37 +110,8 @@ float bf16_to_float(uint16_t h)
     return f;
 }
 
-float round_down(double d)
-{
-    float f = (float)d;
-    if ((double)f > d) f = std::nextafter(f, -INFINITY);
-    return f;
-}
-
-float round_up(double d)
-{
-    float f = (float)d;
-    if ((double)f < d) f = std::nextafter(f, INFINITY);
-    return f;
-}
-
 // gamma_n = n u / (1 - n u): bound factor for any summation order of n terms.
 double gamma_n(double n, double u) { return n * u / (1.0 - n * u); }
-
-// Per-slot thresholds as the kernels use them, from one column threshold.
-struct SlotThr {
-    float lo, thr, seq_lo, seq_hi;
-};
-
-SlotThr slot_thresholds(float t, double band, double seq)
-{
-    SlotThr s;
-    s.thr = t;
-    s.lo = round_down((double)t - band);
-    s.seq_lo = round_down((double)t - seq);
-    s.seq_hi = round_up((double)t + seq);
-    return s;
-}
 
 struct GroupInfo {
     uint32_t blk0, nch, nb;
@@ -174,7 +145,8 @@ struct blamm_hip_ctx {
     std::vector<double> band_h, seq_h;         // per slot
     double max_filter_band = 0.0, max_score_error = 0.0;
     DevBuf<uint8_t> frag_a, frag_b;
-    DevBuf<float> thr_lo, slot_thr, slot_seq_lo, slot_seq_hi, slot_w;
+    DevBuf<float> slot_thr, slot_w;
+    DevBuf<double> slot_band, slot_seq;
     DevBuf<int> slot_len;
     DevBuf<uint32_t> slot_col, groups_d;
 
@@ -316,8 +288,7 @@ int relayout(blamm_hip_ctx* c)
     const int nslots = nblk * cols_per_blk;
     if (nblk >= (1 << 20)) return fail(BLAMM_ERR_ARG, "set_library: too many columns");
 
-    std::vector<float> thr_lo(nslots, INFINITY), s_thr(nslots, INFINITY), s_lo(nslots, INFINITY),
-        s_hi(nslots, INFINITY), s_w((size_t)nslots * kMaxLen * 4, 0.f);
+    std::vector<float> s_thr(nslots, INFINITY), s_w((size_t)nslots * kMaxLen * 4, 0.f);
     std::vector<int> s_len(nslots, 1), s_colh(nslots, -1);
     std::vector<uint32_t> s_col(nslots, 0);
     std::vector<double> s_band(nslots, 0.0), s_seq(nslots, 0.0);
@@ -366,11 +337,7 @@ int relayout(blamm_hip_ctx* c)
             const double band = 2.0 * (rep_err + gamma_n(nterms, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
             // Sequential FP32 band used by finalize (RNE adds, L-1 of them).
             const double seq = 2.0 * (gamma_n(L, u24) * sum_max + fp64_err) + 1e-30;
-            const SlotThr st = slot_thresholds(c->lib_thr[col], band, seq);
-            thr_lo[slot] = st.lo;
-            s_thr[slot] = st.thr;
-            s_lo[slot] = st.seq_lo;
-            s_hi[slot] = st.seq_hi;
+            s_thr[slot] = c->lib_thr[col];
             s_len[slot] = L;
             s_col[slot] = (uint32_t)col;
             s_colh[slot] = col;
@@ -421,20 +388,18 @@ int relayout(blamm_hip_ctx* c)
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(c->frag_a.reserve(fa.size()));
     HIP_TRY(c->frag_b.reserve(fb.size()));
-    HIP_TRY(c->thr_lo.reserve(nslots));
     HIP_TRY(c->slot_thr.reserve(nslots));
-    HIP_TRY(c->slot_seq_lo.reserve(nslots));
-    HIP_TRY(c->slot_seq_hi.reserve(nslots));
+    HIP_TRY(c->slot_band.reserve(nslots));
+    HIP_TRY(c->slot_seq.reserve(nslots));
     HIP_TRY(c->slot_w.reserve(s_w.size()));
     HIP_TRY(c->slot_len.reserve(nslots));
     HIP_TRY(c->slot_col.reserve(nslots));
     HIP_TRY(c->groups_d.reserve(gpk.size()));
     HIP_TRY(hipMemcpy(c->frag_a.p, fa.data(), fa.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->frag_b.p, fb.data(), fb.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->thr_lo.p, thr_lo.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_thr.p, s_thr.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->slot_seq_lo.p, s_lo.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->slot_seq_hi.p, s_hi.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_band.p, s_band.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_seq.p, s_seq.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_w.p, s_w.data(), s_w.size() * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_len.p, s_len.data(), nslots * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_col.p, s_col.data(), nslots * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -452,7 +417,7 @@ int relayout(blamm_hip_ctx* c)
     return BLAMM_OK;
 }
 
-FilterArgs filter_args(const blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, const float* thr_lo,
+FilterArgs filter_args(const blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, const float* thr,
                        uint2* cand, uint64_t cand_cap, unsigned long long* counters)
 {
     FilterArgs a;
@@ -460,12 +425,13 @@ FilterArgs filter_args(const blamm_hip_ctx* c, const uint64_t* packed, const uin
     a.lcmask = lcmask;
     a.frag_a = c->frag_a.p;
     a.frag_b = c->frag_b.p;
-    a.thr_lo = thr_lo;
+    a.thr = thr;
+    a.band = c->slot_band.p;
     a.groups = c->groups_d.p;
     a.cand = cand;
     a.counters = counters;
     a.cand_cap = cand_cap;
-    a.req_stride = 4u * (uint32_t)c->nslots;
+    a.req_stride = (uint32_t)c->nslots;
     return a;
 }
 
@@ -485,12 +451,11 @@ FinalizeArgs finalize_args(const blamm_hip_ctx* c, const uint2* cand, uint64_t c
     f.slot_len = c->slot_len.p;
     f.slot_w = c->slot_w.p;
     f.slot_thr = c->slot_thr.p;
-    f.slot_seq_lo = c->slot_seq_lo.p;
-    f.slot_seq_hi = c->slot_seq_hi.p;
+    f.slot_seq = c->slot_seq.p;
     f.slot_col = c->slot_col.p;
     f.hits = nullptr;
     f.hit_cap = 0;
-    f.req_stride = 4u * (uint32_t)c->nslots;
+    f.req_stride = (uint32_t)c->nslots;
     f.desc = nullptr;
     f.req_end = nullptr;
     return f;
@@ -560,8 +525,8 @@ int blamm_hip_destroy(blamm_hip_ctx* c)
     if (!c) return BLAMM_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    c->frag_a.release(); c->frag_b.release(); c->thr_lo.release(); c->slot_thr.release();
-    c->slot_seq_lo.release(); c->slot_seq_hi.release(); c->slot_w.release();
+    c->frag_a.release(); c->frag_b.release(); c->slot_thr.release(); c->slot_band.release();
+    c->slot_seq.release(); c->slot_w.release();
     c->slot_len.release(); c->slot_col.release(); c->groups_d.release(); c->cand.release();
     c->counters.release();
     c->packed.release(); c->lcmask.release(); c->runs.release(); c->nhits_dev.release();
@@ -677,7 +642,7 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     HIP_TRY(hipEventRecord(c->ev[0], s));
     if (nbases > 0) {
         const uint32_t ntiles = (uint32_t)((nbases + kBf16Tile - 1) / kBf16Tile);   // 32-window tiles
-        const FilterArgs a = filter_args(c, d_packed, d_lcmask, c->thr_lo.p, c->cand.p, c->cand.n, c->counters.p);
+        const FilterArgs a = filter_args(c, d_packed, d_lcmask, c->slot_thr.p, c->cand.p, c->cand.n, c->counters.p);
         WorkQueue q{};
         q.ntiles = ntiles;
         q.chunk_tiles = std::max<uint32_t>(1, std::min<uint32_t>(c->tuning.chunk_tiles, ntiles));

@@ -67,14 +67,34 @@ struct FilterArgs {
     const uint64_t* lcmask;      // lowercase bits (MASK variants only)
     const void* frag_a;          // bf16: hi fragments; fp32: weights
     const void* frag_b;          // bf16: lo fragments
-    const float* thr_lo;         // per slot: filter threshold (thr - band, rounded down);
-                                 // list mode: request r at thr_lo + r * req_stride
+    const float* thr;            // per slot: threshold; list mode: request r at thr + r * req_stride
+    const double* band;          // per slot: filter band (MFMA score vs FP64 score)
     const uint32_t* groups;      // column groups, see group_pack()
     uint2* cand;                 // (window, slot | request << 16 in list mode)
     unsigned long long* counters;
     uint64_t cand_cap;
     uint32_t req_stride;         // list mode: floats between two requests' tables
 };
+
+// Largest float <= t - b and smallest float >= t + b (b >= 0), for the
+// filter and rescoring thresholds.  The double result is pushed a few of its
+// own ulps outward before the directed conversion, which covers the rounding
+// of the double subtraction itself.  Infinite thresholds stay as they are.
+__device__ __forceinline__ float thr_minus(float t, double b)
+{
+    if (!__builtin_isfinite(t)) return t;
+    double d = (double)t - b;
+    d -= __builtin_fabs(d) * 0x1p-50 + 0x1p-1000;
+    return __double2float_rd(d);
+}
+
+__device__ __forceinline__ float thr_plus(float t, double b)
+{
+    if (!__builtin_isfinite(t)) return t;
+    double d = (double)t + b;
+    d += __builtin_fabs(d) * 0x1p-50 + 0x1p-1000;
+    return __double2float_ru(d);
+}
 
 // A column group is nb consecutive blocks of one length class (nch chunks of
 // 4 positions), scored together by one wave.
@@ -272,7 +292,8 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
             bh[b][j] = __builtin_bit_cast(bf16x8, h);
             bl[b][j] = __builtin_bit_cast(bf16x8, l);
         }
-        th[b] = thr[(blk0 + b) * kBf16Cols + (lane & 31)];
+        const uint32_t slot = (blk0 + b) * kBf16Cols + (lane & 31);
+        th[b] = thr_minus(thr[slot], a.band[slot]);
     }
 
     const uint32_t off = (lane & 31) + 2 * (lane >> 5);   // first base of the lane's window, 0..33
@@ -393,7 +414,8 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
     for (int b = 0; b < NB; ++b) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) bw[b][p] = fw[((size_t)(blk0 + b) * kMaxLen + p) * 64 + lane];
-        th[b] = thr[(blk0 + b) * kFp32Cols + (lane & 15)];
+        const uint32_t slot = (blk0 + b) * kFp32Cols + (lane & 15);
+        th[b] = thr_minus(thr[slot], a.band[slot]);
     }
     const uint32_t i = lane & 15, k = lane >> 4;
     const uint32_t s0 = 2 * i, s1 = 2 * (i + 16);    // bit offsets of the two sub-tiles
@@ -537,7 +559,7 @@ __global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs 
             j = __builtin_amdgcn_readfirstlane(__shfl(j, 0));
             if (j >= items) break;
             uint32_t grp, t0, t1, rtag = 0;
-            const float* thr = a.thr_lo;
+            const float* thr = a.thr;
             if constexpr (LIST) {
                 const uint2 it = l.items[base + j];
                 const uint4 ch = l.chunks[__builtin_amdgcn_readfirstlane(it.x)];
@@ -546,7 +568,7 @@ __global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs 
                 t1 = __builtin_amdgcn_readfirstlane(ch.y);
                 const uint32_t req = __builtin_amdgcn_readfirstlane(ch.z);
                 rtag = req << 16;
-                thr = a.thr_lo + (size_t)req * a.req_stride;
+                thr = a.thr + (size_t)req * a.req_stride;
             } else {
                 const uint32_t k = (j / q.ngroups) * kXcds + x;
                 grp = __builtin_amdgcn_readfirstlane(a.groups[j % q.ngroups]);
@@ -571,9 +593,8 @@ struct FinalizeArgs {
     uint32_t nruns;
     const int* slot_len;
     const float* slot_w;                  // [slot][kMaxLen][4] FP32 weights
-    const float* slot_thr;                // the threshold itself
-    const float* slot_seq_lo;             // thr - sequential-FP32 band (rounded down)
-    const float* slot_seq_hi;             // thr + sequential-FP32 band (rounded up)
+    const float* slot_thr;                // the threshold; list form: per request (req_stride)
+    const double* slot_seq;               // sequential-FP32 band per slot
     const uint32_t* slot_col;             // caller's column index
     void* hits;                           // blamm_hit[]
     uint64_t hit_cap;
@@ -682,8 +703,9 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             ml >>= 1;
         }
         const float thr = have ? a.slot_thr[toff] : 0.f;
-        const float shi = have ? a.slot_seq_hi[toff] : 0.f;
-        const float slo = have ? a.slot_seq_lo[toff] : 0.f;
+        const double seq = have ? a.slot_seq[slot] : 0.0;
+        const float shi = thr_plus(thr, seq);
+        const float slo = thr_minus(thr, seq);
         const uint32_t col = have ? a.slot_col[slot] : 0u;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
@@ -743,6 +765,127 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
 __global__ void zero_counters_kernel(unsigned long long* c)
 {
     if (threadIdx.x < kCntSlots) c[threadIdx.x] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// Hits of a batch grouped by request on the device (count, scan, scatter), so
+// the host receives each request's hits contiguously with request-local
+// positions.  Per-request counts and range reservations are taken per block
+// in LDS when the batch has at most kGroupLds requests (one global atomic per
+// block and request instead of one per hit).
+// ---------------------------------------------------------------------------
+constexpr int kGroupLds = 4096;
+constexpr int kGroupThreads = 256;
+
+struct GroupArgs {
+    const BatchDesc* desc;
+    const unsigned long long* counters;   // kCntHits
+    uint64_t cap;                         // hit buffer capacity
+    const DevHit* in;                     // finalize output, pos = request << 32 | batch position
+    DevHit* out;                          // grouped, request-local positions
+    uint32_t* req_count;                  // [nreq], cleared by group_clear_kernel
+    unsigned long long* req_off;          // [nreq + 1] exclusive offsets
+    unsigned long long* cursor;           // [nreq] next free slot per request
+    const uint32_t* req_start;            // [nreq] first batch position of each request
+};
+
+__global__ __launch_bounds__(kGroupThreads) void group_clear_kernel(GroupArgs g)
+{
+    const uint32_t nreq = g.desc->nreq;
+    for (uint32_t i = blockIdx.x * kGroupThreads + threadIdx.x; i < nreq; i += gridDim.x * kGroupThreads)
+        g.req_count[i] = 0u;
+}
+
+__device__ __forceinline__ uint64_t group_nhits(const GroupArgs& g)
+{
+    const uint64_t n = g.counters[kCntHits];
+    return n < g.cap ? n : g.cap;
+}
+
+__global__ __launch_bounds__(kGroupThreads) void group_count_kernel(GroupArgs g)
+{
+    __shared__ uint32_t cnt[kGroupLds];
+    const uint32_t nreq = g.desc->nreq;
+    const uint64_t n = group_nhits(g);
+    const bool lds = nreq <= (uint32_t)kGroupLds;
+    if (lds)
+        for (uint32_t i = threadIdx.x; i < nreq; i += kGroupThreads) cnt[i] = 0u;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * kGroupThreads + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kGroupThreads) {
+        const uint32_t r = (uint32_t)(g.in[i].pos >> 32);
+        if (lds) atomicAdd(&cnt[r], 1u);
+        else atomicAdd(&g.req_count[r], 1u);
+    }
+    __syncthreads();
+    if (lds)
+        for (uint32_t i = threadIdx.x; i < nreq; i += kGroupThreads)
+            if (cnt[i]) atomicAdd(&g.req_count[i], cnt[i]);
+}
+
+// One block: exclusive prefix over the request counts.
+__global__ __launch_bounds__(1024) void group_scan_kernel(GroupArgs g)
+{
+    __shared__ unsigned long long part[1024];
+    const uint32_t nreq = g.desc->nreq;
+    const uint32_t per = (nreq + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per, e = min(nreq, b + per);
+    unsigned long long s = 0;
+    for (uint32_t i = b; i < e; ++i) s += g.req_count[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const unsigned long long v = threadIdx.x >= d ? part[threadIdx.x - d] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long off = part[threadIdx.x] - s;   // exclusive
+    for (uint32_t i = b; i < e; ++i) {
+        g.req_off[i] = off;
+        g.cursor[i] = off;
+        off += g.req_count[i];
+    }
+    if (threadIdx.x == 1023) g.req_off[nreq] = part[1023];
+}
+
+// Each block scatters a contiguous range of hits: counts per request in LDS,
+// reserves one global range per request, then places hits by LDS atomics.
+__global__ __launch_bounds__(kGroupThreads) void group_scatter_kernel(GroupArgs g)
+{
+    __shared__ uint32_t cnt[kGroupLds];
+    __shared__ unsigned long long base[kGroupLds];
+    const uint32_t nreq = g.desc->nreq;
+    const uint64_t n = group_nhits(g);
+    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t b0 = (uint64_t)blockIdx.x * per;
+    const uint64_t b1 = b0 + per < n ? b0 + per : n;
+    if (nreq > (uint32_t)kGroupLds) {
+        for (uint64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) {
+            DevHit h = g.in[i];
+            const uint32_t r = (uint32_t)(h.pos >> 32);
+            const unsigned long long at = atomicAdd(&g.cursor[r], 1ull);
+            h.pos = (h.pos & 0xFFFFFFFFull) - g.req_start[r];
+            g.out[at] = h;
+        }
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < nreq; i += kGroupThreads) cnt[i] = 0u;
+    __syncthreads();
+    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) atomicAdd(&cnt[(uint32_t)(g.in[i].pos >> 32)], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nreq; i += kGroupThreads) {
+        base[i] = cnt[i] ? atomicAdd(&g.cursor[i], (unsigned long long)cnt[i]) : 0ull;
+        cnt[i] = 0u;
+    }
+    __syncthreads();
+    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) {
+        DevHit h = g.in[i];
+        const uint32_t r = (uint32_t)(h.pos >> 32);
+        const unsigned long long at = base[r] + atomicAdd(&cnt[r], 1u);
+        h.pos = (h.pos & 0xFFFFFFFFull) - g.req_start[r];
+        g.out[at] = h;
+    }
 }
 static_assert(kCntQueue + kXcds <= kCntSlots, "queue counters");
 

@@ -12,6 +12,7 @@
 
 #include <condition_variable>
 #include <deque>
+#include <memory>
 #include <thread>
 #include <unordered_map>
 
@@ -30,7 +31,8 @@ struct ServeRequest {
     uint64_t bytes = 0;              // admission estimate
     int status = 1;                  // 1 = queued/running, else final status
     std::string error;
-    std::vector<blamm_hit> hits;
+    std::shared_ptr<HitBuf> buf;                   // the batch's hits, grouped by request
+    uint64_t off = 0, nhits = 0;                   // this request's slice of buf
     uint64_t t_submit = 0, t_start = 0, t_done = 0;
     uint32_t batch_requests = 0;
 };
@@ -79,27 +81,25 @@ void serve_finish(blamm_hip_server* S, ServeSlot& sl, int rc, std::unique_lock<s
 {
     // called with the lock held; the batch has been waited for (rc)
     const uint64_t t = now_ns();
+    // the requests share the batch's grouped hits (no per-request copy)
+    std::shared_ptr<HitBuf> buf;
+    if (rc == BLAMM_OK) buf = sl.batch->out;
     for (uint32_t r = 0; r < (uint32_t)sl.ids.size(); ++r) {
         auto it = S->reqs.find(sl.ids[r]);
         if (it == S->reqs.end()) continue;
         ServeRequest& q = it->second;
         if (rc == BLAMM_OK) {
-            const blamm_hit* h = nullptr;
-            uint64_t n = 0;
-            if (blamm_hip_batch_hits(sl.batch, r, &h, &n) == BLAMM_OK) {
-                q.hits.assign(h, h + n);
-                q.status = BLAMM_OK;
-            } else {
-                q.status = BLAMM_ERR_STATE;
-                q.error = blamm_hip_last_error();
-            }
+            q.buf = buf;
+            q.off = sl.batch->out_off[r];
+            q.nhits = sl.batch->out_off[r + 1] - sl.batch->out_off[r];
+            q.status = BLAMM_OK;
         } else {
             q.status = rc;
             q.error = blamm_hip_last_error();
         }
         q.t_done = t;
         S->stats.completed++;
-        S->stats.hits += q.hits.size();
+        S->stats.hits += q.nhits;
     }
     sl.ids.clear();
     sl.busy = false;
@@ -277,7 +277,7 @@ int blamm_hip_server_result(blamm_hip_server* S, uint64_t id, blamm_hit* hits, u
     if (q.status == 1) return fail(BLAMM_ERR_STATE, "server_result: request not finished");
     if (info) {
         info->status = q.status;
-        info->nhits = q.hits.size();
+        info->nhits = q.nhits;
         info->t_submit_ns = q.t_submit;
         info->t_start_ns = q.t_start;
         info->t_done_ns = q.t_done;
@@ -289,13 +289,34 @@ int blamm_hip_server_result(blamm_hip_server* S, uint64_t id, blamm_hit* hits, u
         S->reqs.erase(it);
         return fail(st, "request failed: " + msg);
     }
-    if (q.hits.size() > cap) return fail(BLAMM_ERR_OVERFLOW, "server_result: hit capacity too small");
-    if (!q.hits.empty()) {
+    if (q.nhits > cap) return fail(BLAMM_ERR_OVERFLOW, "server_result: hit capacity too small");
+    if (q.nhits) {
         if (!hits) return fail(BLAMM_ERR_ARG, "server_result: null hit buffer");
-        std::memcpy(hits, q.hits.data(), q.hits.size() * sizeof(blamm_hit));
+        std::memcpy(hits, q.buf->p + q.off, q.nhits * sizeof(blamm_hit));
     }
     S->reqs.erase(it);
     return BLAMM_OK;
+}
+
+int blamm_hip_server_release(blamm_hip_server* S, uint64_t id, blamm_request_info* info)
+{
+    if (!S) return fail(BLAMM_ERR_ARG, "server_release: null server");
+    std::lock_guard<std::mutex> g(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(BLAMM_ERR_ARG, "server_release: unknown request");
+    ServeRequest& q = it->second;
+    if (q.status == 1) return fail(BLAMM_ERR_STATE, "server_release: request not finished");
+    if (info) {
+        info->status = q.status;
+        info->nhits = q.nhits;
+        info->t_submit_ns = q.t_submit;
+        info->t_start_ns = q.t_start;
+        info->t_done_ns = q.t_done;
+        info->batch_requests = q.batch_requests;
+    }
+    const int st = q.status;
+    S->reqs.erase(it);
+    return st == BLAMM_OK ? BLAMM_OK : fail(st, "request failed");
 }
 
 int blamm_hip_server_stats(blamm_hip_server* S, blamm_server_stats* st)

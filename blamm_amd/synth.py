@@ -154,3 +154,49 @@ def counts_of(codes: np.ndarray) -> np.ndarray:
 
 def reverse_complement_pfm(pfm: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(pfm[::-1, ::-1])
+
+
+def config2(seed: int = 2, n_each: int = 62_500_000, groups: int = 4):
+    """Precision-form workload: 2,000 motifs (L 5..30, Dirichlet 0.3) and
+    `groups` records of n_each bases, each its own manifest group with GC drawn
+    uniformly from 0.35..0.50.  Returns (motifs, [(name, codes, gc)])."""
+    rng = np.random.default_rng(seed)
+    m = motifs(2000, 5, 30, 0.3, seed=seed)
+    recs = []
+    for i in range(groups):
+        gc = float(rng.uniform(0.35, 0.50))
+        recs.append((f"grp{i}", bases(n_each, gc, rng), gc))
+    return m, recs
+
+
+def config3(seed: int = 3, nseq: int = 20_000):
+    """Batched uneven workload: nseq records with lengths lognormal (median
+    2 kbp, sigma 1.0, clipped to 200 bp..200 kbp) at GC 0.45 and 1,500 motifs
+    (L 6..25).  Returns (motifs, codes, run_starts, lengths): the records as
+    one filtered stream whose runs are the records."""
+    rng = np.random.default_rng(seed)
+    m = motifs(1500, 6, 25, 0.5, seed=seed)
+    lengths = np.clip(np.rint(rng.lognormal(np.log(2000.0), 1.0, size=nseq)), 200, 200_000).astype(np.int64)
+    codes = bases(int(lengths.sum()), 0.45, rng)
+    starts = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
+    return m, codes, starts, lengths
+
+
+def config4_library(seed: int = 4) -> MotifSet:
+    """Serving library: 2,000 motifs drawn like the precision workload's."""
+    return motifs(2000, 5, 30, 0.3, seed=seed)
+
+
+def config4_request(rng: np.random.Generator, nmotifs: int, gc: float = 0.41, max_seqs: int = 5000):
+    """One serving request: 1..max_seqs sequences with lengths lognormal
+    (median 1 kbp, sigma 1.0, at most 1 Mbp), a motif subset of 10..2,000 drawn
+    uniformly from the library, and a p-value from {1e-3, 1e-4, 1e-5, 1e-6}.
+    Returns (codes, run_starts, motif_subset, pvalue)."""
+    nseq = int(rng.integers(1, max_seqs + 1))
+    lengths = np.clip(np.rint(rng.lognormal(np.log(1000.0), 1.0, size=nseq)), 1, 1_000_000).astype(np.int64)
+    codes = bases(int(lengths.sum()), gc, rng)
+    starts = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.uint64)
+    k = int(rng.integers(10, min(2000, nmotifs) + 1))
+    subset = np.sort(rng.choice(nmotifs, size=k, replace=False))
+    pvalue = float(rng.choice([1e-3, 1e-4, 1e-5, 1e-6]))
+    return codes, starts, subset, pvalue

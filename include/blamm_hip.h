@@ -270,6 +270,9 @@ int blamm_hip_server_wait(blamm_hip_server *server, uint64_t id, int64_t timeout
  * below info->nhits. */
 int blamm_hip_server_result(blamm_hip_server *server, uint64_t id, blamm_hit *hits, uint64_t cap,
                             blamm_request_info *info);
+/* Report a finished request's timing and hit count, drop its hits and forget
+ * it (for callers that only need the count). */
+int blamm_hip_server_release(blamm_hip_server *server, uint64_t id, blamm_request_info *info);
 int blamm_hip_server_stats(blamm_hip_server *server, blamm_server_stats *stats);
 /* Finishes queued and running requests, then stops the worker. */
 int blamm_hip_server_destroy(blamm_hip_server *server);
