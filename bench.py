@@ -364,7 +364,9 @@ def serve_leg(args):
         t = np.full(ncols, np.inf, np.float32)
         cols = np.concatenate([2 * subset, 2 * subset + 1])
         t[cols] = thr[p][cols]
-        req = ba.ScanRequest(ba.pack_codes(codes), int(codes.size), starts, None, t)
+        # admission hint: the histograms' cutoffs pass a few times p per window
+        hint = int(3 * p * codes.size * cols.size) + 1024
+        req = ba.ScanRequest(ba.pack_codes(codes), int(codes.size), starts, None, t, expected_hits=hint)
         pool.append((req, float(codes.size) * cols.size))
     log(f"serve workload: library {ncols} columns, pool of {len(pool)} requests, "
         f"{sum(r.nbases for r, _ in pool) / len(pool) / 1e6:.2f} Mbp per request ({time.time() - t0:.1f}s)")
@@ -403,9 +405,11 @@ def serve_leg(args):
         span = (t_last - t_first) / 1e9
         return np.asarray(lat), work, span, float(np.mean(batch))
 
-    run([0.0] * 4)   # warm-up (captures the slot graphs)
+    run([0.0] * len(pool))   # warm-up: captures the slot graphs, sizes scratch and pinned buffers
     n_closed = 2 * len(pool)
+    s0 = srv.stats()
     lat, work, span, mb = run([0.0] * n_closed)
+    s1 = srv.stats()
     cap_rps = n_closed / span
     out = {
         "workload": f"serving config: library {ncols // 2} motifs (L 5-30, Dirichlet 0.3) fwd+rc = {ncols} columns; "
@@ -414,7 +418,11 @@ def serve_leg(args):
         "unit": "Gbp*motifs/s",
         "closed_loop": {"requests": n_closed, "rps": cap_rps, "value": work / span / 1e9,
                         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                        "mean_batch": mb},
+                        "mean_batch": mb, "span_ms": span * 1e3,
+                        "device_ms": s1.device_ms - s0.device_ms,
+                        "host_submit_ms": s1.host_submit_ms - s0.host_submit_ms,
+                        "host_wait_ms": s1.host_wait_ms - s0.host_wait_ms,
+                        "replays": int(s1.replays - s0.replays)},
         "poisson": [],
     }
     for f in args.serve_loads:
@@ -428,7 +436,8 @@ def serve_leg(args):
                                "mean_batch": mb})
         log(f"serve load {f:.2f}: {out['poisson'][-1]}")
     st = srv.stats()
-    out["batches"] = int(st.batches)
+    out.update({"batches": int(st.batches), "replays": int(st.replays), "device_ms": st.device_ms,
+                "host_submit_ms": st.host_submit_ms, "host_wait_ms": st.host_wait_ms})
     srv.close()
     sc.close()
     return out

@@ -79,7 +79,8 @@ class RequestInfo(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
-                ("hits", ctypes.c_uint64), ("device_ms", ctypes.c_double)]
+                ("hits", ctypes.c_uint64), ("replays", ctypes.c_uint64), ("device_ms", ctypes.c_double),
+                ("host_submit_ms", ctypes.c_double), ("host_wait_ms", ctypes.c_double)]
 
 
 _hip = None

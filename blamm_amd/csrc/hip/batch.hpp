@@ -77,7 +77,13 @@ struct blamm_hip_batch {
 
     // device buffers
     DevBuf<uint64_t> packed, lcmask, runs;
-    DevBuf<float> tab;              // [nreq][nslots] thresholds
+    // the batch's own column blocks: each request's selected columns
+    DevBuf<float> thr;              // per batch slot: the request's threshold
+    DevBuf<uint32_t> slot_map;      // per batch slot: library slot (kNoSlot = empty)
+    DevBuf<uint32_t> blk_req;       // per batch block: request
+    DevBuf<uint32_t> groups;        // column groups (group_pack)
+    DevBuf<uint4> frag_a, frag_b;   // gathered fragments (fp32 weights in frag_a)
+    DevBuf<double> band;            // per batch slot
     DevBuf<uint32_t> req_start, req_end, req_count;
     DevBuf<unsigned long long> req_off, cursor;
     DevBuf<uint2> items, cand;
@@ -105,6 +111,7 @@ struct blamm_hip_batch {
     std::vector<uint64_t> start_h, n_h;
     std::vector<uint64_t> out_off;
     blamm_scan_stats stats{};
+    uint64_t replays = 0;
 };
 
 namespace {
@@ -134,19 +141,45 @@ int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
     const bool bf = c->precision == BLAMM_PREC_BF16X2;
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, b->stream, b->counters.p);
     HIP_TRY(hipGetLastError());
-    const FilterArgs a = filter_args(c, b->packed.p, mask ? b->lcmask.p : nullptr, b->tab.p, b->cand.p, b->cand.n,
-                                     b->counters.p);
+    // the batch's column blocks from the library's fragments
+    GatherArgs ga;
+    ga.desc = b->desc.p;
+    ga.slot_map = b->slot_map.p;
+    ga.lib_a = (const uint4*)c->frag_a.p;
+    ga.lib_b = (const uint4*)c->frag_b.p;
+    ga.lib_w = (const float*)c->frag_a.p;
+    ga.lib_band = c->slot_band_any.p;
+    ga.out_a = b->frag_a.p;
+    ga.out_b = b->frag_b.p;
+    ga.out_w = (float*)b->frag_a.p;
+    ga.out_band = b->band.p;
+    if (bf) hipLaunchKernelGGL(gather_kernel<true>, dim3((uint32_t)c->num_cu * 4), dim3(256), 0, b->stream, ga);
+    else hipLaunchKernelGGL(gather_kernel<false>, dim3((uint32_t)c->num_cu * 4), dim3(256), 0, b->stream, ga);
+    HIP_TRY(hipGetLastError());
+    FilterArgs a;
+    a.packed = b->packed.p;
+    a.lcmask = mask ? b->lcmask.p : nullptr;
+    a.frag_a = b->frag_a.p;
+    a.frag_b = b->frag_b.p;
+    a.thr = b->thr.p;
+    a.band = b->band.p;
+    a.groups = b->groups.p;
+    a.cand = b->cand.p;
+    a.counters = b->counters.p;
+    a.cand_cap = b->cand.n;
     const WorkQueue q{0, 0, 1, 1};
     const ListArgs l{b->desc.p, b->items.p, b->chunks.p};
     hipError_t e = launch_filter(bf, mask, true, a, q, l, grid, b->stream);
     if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch filter launch: ") + hipGetErrorString(e));
     FinalizeArgs f = finalize_args(c, b->cand.p, b->cand.n, b->counters.p, b->packed.p, mask ? b->lcmask.p : nullptr,
                                    b->runs.p);
-    f.slot_thr = b->tab.p;
+    f.slot_thr = b->thr.p;
     f.hits = b->hits.p;
     f.hit_cap = b->hits.n;
     f.desc = b->desc.p;
     f.req_end = b->req_end.p;
+    f.slot_map = b->slot_map.p;
+    f.blk_req = b->blk_req.p;
     e = launch_finalize(bf, true, f, (uint32_t)c->num_cu * 8, b->stream);
     if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch finalize launch: ") + hipGetErrorString(e));
     // group the hits by request on the device
@@ -249,7 +282,9 @@ int blamm_hip_batch_destroy(blamm_hip_batch* b)
     (void)hipSetDevice(b->ctx->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    b->packed.release(); b->lcmask.release(); b->runs.release(); b->tab.release(); b->req_start.release();
+    b->packed.release(); b->lcmask.release(); b->runs.release(); b->thr.release(); b->slot_map.release();
+    b->blk_req.release(); b->groups.release(); b->frag_a.release(); b->frag_b.release(); b->band.release();
+    b->req_start.release();
     b->req_end.release(); b->req_count.release(); b->req_off.release(); b->cursor.release();
     b->items.release(); b->cand.release(); b->chunks.release(); b->desc.release(); b->counters.release();
     b->hits.release(); b->grouped.release();
@@ -266,7 +301,8 @@ int blamm_hip_batch_destroy(blamm_hip_batch* b)
 uint64_t blamm_hip_batch_device_bytes(const blamm_hip_batch* b)
 {
     if (!b) return 0;
-    return b->packed.n * 8 + b->lcmask.n * 8 + b->runs.n * 8 + b->tab.n * 4 +
+    return b->packed.n * 8 + b->lcmask.n * 8 + b->runs.n * 8 + b->thr.n * 4 + b->slot_map.n * 4 + b->blk_req.n * 4 +
+           b->groups.n * 4 + (b->frag_a.n + b->frag_b.n) * 16 + b->band.n * 8 +
            (b->req_start.n + b->req_end.n + b->req_count.n) * 4 + (b->req_off.n + b->cursor.n) * 8 + b->items.n * 8 +
            b->cand.n * 8 + b->chunks.n * 16 + (b->hits.n + b->grouped.n) * sizeof(blamm_hit) + sizeof(BatchDesc) +
            kCntSlots * 8;
@@ -280,7 +316,6 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     if (b->pending) return fail(BLAMM_ERR_STATE, "batch_submit: previous batch not waited for");
     if (nreq == 0 || !reqs) return fail(BLAMM_ERR_ARG, "batch_submit: no requests");
     if (nreq >= (1u << 16)) return fail(BLAMM_ERR_ARG, "batch_submit: at most 65535 requests per batch");
-    if (c->nslots >= (1 << 16)) return fail(BLAMM_ERR_ARG, "batch_submit: library too large for batches (>= 65536 slots)");
     HIP_TRY(hipSetDevice(c->device));
 
     // Layout of the batch stream: request r occupies [start_r, start_r + n_r),
@@ -309,38 +344,77 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     }
     const uint64_t pw = total / 32 + 16, mw = mask ? total / 64 + 16 : 0;
     const size_t nslots = (size_t)c->nslots;
-    const int cpb = c->cols_per_blk;
+    const uint32_t cpb = (uint32_t)c->cols_per_blk;
     const uint32_t ct = std::max<uint32_t>(1, c->tuning.chunk_tiles);
+    const bool bf = c->precision == BLAMM_PREC_BF16X2;
+    const uint32_t nbmax = std::min<uint32_t>(c->tuning.blocks_per_wave, bf ? kNbMaxBf16 : kNbMaxFp32);
+    const int kstep = (int)c->tuning.kpad_positions / 4;
 
-    // Thresholds in slot order and active column groups per request.
-    std::vector<std::vector<uint32_t>> active(nreq);
-    std::vector<float> tabh((size_t)nreq * nslots, INFINITY);
+    // Each request's selected columns (finite or -inf threshold), in library
+    // slot order (non-decreasing length), packed into blocks of its own.
+    std::vector<uint32_t> smap, breq, gpk;
+    std::vector<float> sthr;
+    std::vector<std::vector<uint32_t>> rgroups(nreq);
     for (uint32_t r = 0; r < nreq; ++r) {
         const float* thr = reqs[r].thresholds ? reqs[r].thresholds : c->lib_thr.data();
-        float* t0 = &tabh[(size_t)r * nslots];
-        for (size_t s = 0; s < nslots; ++s) {
-            const int col = c->slot_col_h[s];
-            if (col >= 0) t0[s] = thr[col];
+        std::vector<uint32_t> sel;
+        for (size_t sl = 0; sl < nslots; ++sl) {
+            const int col = c->slot_col_h[sl];
+            if (col >= 0 && thr[col] != INFINITY) sel.push_back((uint32_t)sl);
         }
-        for (uint32_t g = 0; g < (uint32_t)c->groups.size(); ++g) {
-            const GroupInfo& gi = c->groups[g];
-            bool any = false;
-            for (size_t s = (size_t)gi.blk0 * cpb; s < (size_t)(gi.blk0 + gi.nb) * cpb && !any; ++s)
-                any = t0[s] != INFINITY;
-            if (any) active[r].push_back(g);
+        if (sel.empty() || b->n_h[r] == 0) continue;
+        const uint32_t blk_first = (uint32_t)breq.size();
+        std::vector<int> nchs;
+        for (size_t i = 0; i < sel.size(); i += cpb) {
+            int lm = 0;
+            for (uint32_t k = 0; k < cpb; ++k) {
+                if (i + k < sel.size()) {
+                    const uint32_t sl = sel[i + k];
+                    const int col = c->slot_col_h[sl];
+                    smap.push_back(sl);
+                    sthr.push_back(thr[col]);
+                    lm = std::max(lm, c->lib_len[col]);
+                } else {
+                    smap.push_back(kNoSlot);
+                    sthr.push_back(INFINITY);
+                }
+            }
+            int nch = (lm + 3) / 4;
+            nch = std::min(kMaxChunks, (nch + kstep - 1) / kstep * kstep);
+            nchs.push_back(nch);
+            breq.push_back(r);
+        }
+        std::vector<GroupInfo> gr;
+        for (size_t i = 0; i < nchs.size();) {
+            size_t e = i;
+            while (e < nchs.size() && nchs[e] == nchs[i]) ++e;
+            size_t bb = i;
+            for (uint32_t nb = nbmax; nb >= 1; nb /= 2)
+                while (e - bb >= nb) {
+                    gr.push_back({blk_first + (uint32_t)bb, (uint32_t)nchs[i], nb});
+                    bb += nb;
+                }
+            i = e;
+        }
+        std::stable_sort(gr.begin(), gr.end(), [](const GroupInfo& x, const GroupInfo& y) { return x.nch > y.nch; });
+        for (const GroupInfo& g : gr) {
+            rgroups[r].push_back((uint32_t)gpk.size());
+            gpk.push_back(group_pack(g.blk0, g.nch, g.nb));
         }
     }
+    const uint32_t nblk = (uint32_t)breq.size();
+    if (nblk >= (1u << 20)) return fail(BLAMM_ERR_ARG, "batch_submit: too many column blocks in one batch");
     // Chunks and items, chunk k to sub-queue k % kXcds, chunk-major.
     std::vector<uint4> chunks;
     std::vector<std::vector<uint2>> sub(kXcds);
     for (uint32_t r = 0; r < nreq; ++r) {
-        if (b->n_h[r] == 0 || active[r].empty()) continue;
+        if (b->n_h[r] == 0 || rgroups[r].empty()) continue;
         const uint32_t tb = (uint32_t)(b->start_h[r] / kBf16Tile);
         const uint32_t te = (uint32_t)((b->start_h[r] + b->n_h[r] + kBf16Tile - 1) / kBf16Tile);
         for (uint32_t t = tb; t < te; t += ct) {
             const uint32_t k = (uint32_t)chunks.size();
             chunks.push_back(make_uint4(t, std::min(te, t + ct), r, 0));
-            for (uint32_t g : active[r]) sub[k % kXcds].push_back(make_uint2(k, g));
+            for (uint32_t g : rgroups[r]) sub[k % kXcds].push_back(make_uint2(k, g));
         }
     }
     BatchDesc d{};
@@ -354,12 +428,22 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     d.nreq = nreq;
     d.nbases = (uint32_t)total;
     d.nruns = (uint32_t)nruns;
+    d.nblk = nblk;
 
     // Device capacity (growth invalidates captured graphs).
     BLAMM_TRY(batch_grow(b, b->packed, pw));
     if (mask) BLAMM_TRY(batch_grow(b, b->lcmask, mw));
     BLAMM_TRY(batch_grow(b, b->runs, nruns + 1));
-    BLAMM_TRY(batch_grow(b, b->tab, tabh.size()));
+    const size_t nbs = std::max<size_t>(smap.size(), 1);
+    BLAMM_TRY(batch_grow(b, b->thr, nbs));
+    BLAMM_TRY(batch_grow(b, b->slot_map, nbs));
+    BLAMM_TRY(batch_grow(b, b->band, nbs));
+    BLAMM_TRY(batch_grow(b, b->blk_req, std::max<size_t>(nblk, 1)));
+    BLAMM_TRY(batch_grow(b, b->groups, std::max<size_t>(gpk.size(), 1)));
+    const size_t fr = (size_t)std::max<uint32_t>(nblk, 1) * (bf ? (size_t)kMaxChunks * 64 : (size_t)kMaxLen * 16);
+    BLAMM_TRY(batch_grow(b, b->frag_a, fr));
+    if (bf) BLAMM_TRY(batch_grow(b, b->frag_b, fr));
+    else BLAMM_TRY(batch_grow(b, b->frag_b, 1));
     BLAMM_TRY(batch_grow(b, b->req_start, nreq));
     BLAMM_TRY(batch_grow(b, b->req_end, nreq));
     BLAMM_TRY(batch_grow(b, b->req_count, nreq));
@@ -370,14 +454,27 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
         HIP_TRY(b->off_h.reserve((size_t)nreq + 1 + nreq / 8));
     }
     BLAMM_TRY(batch_grow(b, b->items, std::max<size_t>(nitems, 1)));
+    // room for the expected candidates and hits (records <= hits + rescored
+    // windows; 1.5x margin), so the batch rarely needs a second run
+    uint64_t expect = 0;
+    for (uint32_t r = 0; r < nreq; ++r) expect += reqs[r].expected_hits;
+    if (expect) {
+        expect += expect / 2 + 4096;
+        BLAMM_TRY(batch_grow(b, b->cand, expect));
+        BLAMM_TRY(batch_grow(b, b->hits, expect));
+        BLAMM_TRY(batch_grow(b, b->grouped, b->hits.n));
+    }
     BLAMM_TRY(batch_grow(b, b->chunks, std::max<size_t>(chunks.size(), 1)));
 
     // Stage everything in pinned memory, then one copy per buffer.
     const size_t o_pk = 0;
     const size_t o_mk = align_up(o_pk + pw * 8, 256);
     const size_t o_rn = align_up(o_mk + mw * 8, 256);
-    const size_t o_tb = align_up(o_rn + (nruns + 1) * 8, 256);
-    const size_t o_re = align_up(o_tb + tabh.size() * 4, 256);
+    const size_t o_tb = align_up(o_rn + (nruns + 1) * 8, 256);        // thresholds
+    const size_t o_sm = align_up(o_tb + sthr.size() * 4, 256);        // slot map
+    const size_t o_br = align_up(o_sm + smap.size() * 4, 256);        // block requests
+    const size_t o_gp = align_up(o_br + (size_t)nblk * 4, 256);       // groups
+    const size_t o_re = align_up(o_gp + gpk.size() * 4, 256);
     const size_t o_rs = align_up(o_re + (size_t)nreq * 4, 256);
     const size_t o_ch = align_up(o_rs + (size_t)nreq * 4, 256);
     const size_t o_it = align_up(o_ch + chunks.size() * 16, 256);
@@ -410,7 +507,10 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
         rs[r] = (uint32_t)s0;
     }
     d.nruns = (uint32_t)ir;
-    std::memcpy(S + o_tb, tabh.data(), tabh.size() * 4);
+    std::memcpy(S + o_tb, sthr.data(), sthr.size() * 4);
+    std::memcpy(S + o_sm, smap.data(), smap.size() * 4);
+    std::memcpy(S + o_br, breq.data(), (size_t)nblk * 4);
+    std::memcpy(S + o_gp, gpk.data(), gpk.size() * 4);
     std::memcpy(S + o_ch, chunks.data(), chunks.size() * 16);
     size_t at = 0;
     for (int x = 0; x < kXcds; ++x) {
@@ -423,7 +523,12 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     HIP_TRY(hipMemcpyAsync(b->packed.p, S + o_pk, pw * 8, hipMemcpyHostToDevice, s));
     if (mask) HIP_TRY(hipMemcpyAsync(b->lcmask.p, S + o_mk, mw * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(b->runs.p, S + o_rn, std::max<size_t>(ir, 1) * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(b->tab.p, S + o_tb, tabh.size() * 4, hipMemcpyHostToDevice, s));
+    if (!smap.empty()) {
+        HIP_TRY(hipMemcpyAsync(b->thr.p, S + o_tb, sthr.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(b->slot_map.p, S + o_sm, smap.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(b->blk_req.p, S + o_br, (size_t)nblk * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(b->groups.p, S + o_gp, gpk.size() * 4, hipMemcpyHostToDevice, s));
+    }
     HIP_TRY(hipMemcpyAsync(b->req_end.p, S + o_re, (size_t)nreq * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(b->req_start.p, S + o_rs, (size_t)nreq * 4, hipMemcpyHostToDevice, s));
     if (!chunks.empty()) HIP_TRY(hipMemcpyAsync(b->chunks.p, S + o_ch, chunks.size() * 16, hipMemcpyHostToDevice, s));
@@ -470,6 +575,7 @@ int blamm_hip_batch_wait(blamm_hip_batch* b, uint64_t* nhits)
         }
         // Grow what overflowed and replay the batch (its inputs are still on
         // the device).
+        b->replays++;
         if (cand > b->cand.n) BLAMM_TRY(batch_grow(b, b->cand, cand + cand / 4 + 1024));
         if (nh > b->hits.n) {
             BLAMM_TRY(batch_grow(b, b->hits, nh + nh / 4 + 1024));

@@ -146,7 +146,7 @@ struct blamm_hip_ctx {
     double max_filter_band = 0.0, max_score_error = 0.0;
     DevBuf<uint8_t> frag_a, frag_b;
     DevBuf<float> slot_thr, slot_w;
-    DevBuf<double> slot_band, slot_seq;
+    DevBuf<double> slot_band, slot_band_any, slot_seq;   // band_any: valid in a block of any length
     DevBuf<int> slot_len;
     DevBuf<uint32_t> slot_col, groups_d;
 
@@ -291,7 +291,7 @@ int relayout(blamm_hip_ctx* c)
     std::vector<float> s_thr(nslots, INFINITY), s_w((size_t)nslots * kMaxLen * 4, 0.f);
     std::vector<int> s_len(nslots, 1), s_colh(nslots, -1);
     std::vector<uint32_t> s_col(nslots, 0);
-    std::vector<double> s_band(nslots, 0.0), s_seq(nslots, 0.0);
+    std::vector<double> s_band(nslots, 0.0), s_band_any(nslots, 0.0), s_seq(nslots, 0.0);
     std::vector<uint16_t> hi16((size_t)nslots * kMaxLen * 4, 0), lo16((size_t)nslots * kMaxLen * 4, 0);
     double max_band = 0.0, max_err = 0.0;
     const double u24 = std::ldexp(1.0, -24), u53 = std::ldexp(1.0, -53);
@@ -335,6 +335,9 @@ int relayout(blamm_hip_ctx* c)
             // roundoff 2^-22 to allow for truncating accumulation.
             const double nterms = bf ? 32.0 * nch : 16.0 * nch;
             const double band = 2.0 * (rep_err + gamma_n(nterms, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
+            const double nterms_any = bf ? 32.0 * kMaxChunks : 16.0 * kMaxChunks;
+            s_band_any[slot] =
+                2.0 * (rep_err + gamma_n(nterms_any, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
             // Sequential FP32 band used by finalize (RNE adds, L-1 of them).
             const double seq = 2.0 * (gamma_n(L, u24) * sum_max + fp64_err) + 1e-30;
             s_thr[slot] = c->lib_thr[col];
@@ -390,6 +393,7 @@ int relayout(blamm_hip_ctx* c)
     HIP_TRY(c->frag_b.reserve(fb.size()));
     HIP_TRY(c->slot_thr.reserve(nslots));
     HIP_TRY(c->slot_band.reserve(nslots));
+    HIP_TRY(c->slot_band_any.reserve(nslots));
     HIP_TRY(c->slot_seq.reserve(nslots));
     HIP_TRY(c->slot_w.reserve(s_w.size()));
     HIP_TRY(c->slot_len.reserve(nslots));
@@ -399,6 +403,7 @@ int relayout(blamm_hip_ctx* c)
     HIP_TRY(hipMemcpy(c->frag_b.p, fb.data(), fb.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_thr.p, s_thr.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_band.p, s_band.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_band_any.p, s_band_any.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_seq.p, s_seq.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_w.p, s_w.data(), s_w.size() * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_len.p, s_len.data(), nslots * sizeof(int), hipMemcpyHostToDevice));
@@ -431,7 +436,6 @@ FilterArgs filter_args(const blamm_hip_ctx* c, const uint64_t* packed, const uin
     a.cand = cand;
     a.counters = counters;
     a.cand_cap = cand_cap;
-    a.req_stride = (uint32_t)c->nslots;
     return a;
 }
 
@@ -455,9 +459,10 @@ FinalizeArgs finalize_args(const blamm_hip_ctx* c, const uint2* cand, uint64_t c
     f.slot_col = c->slot_col.p;
     f.hits = nullptr;
     f.hit_cap = 0;
-    f.req_stride = (uint32_t)c->nslots;
     f.desc = nullptr;
     f.req_end = nullptr;
+    f.slot_map = nullptr;
+    f.blk_req = nullptr;
     return f;
 }
 
@@ -526,6 +531,7 @@ int blamm_hip_destroy(blamm_hip_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->frag_a.release(); c->frag_b.release(); c->slot_thr.release(); c->slot_band.release();
+    c->slot_band_any.release();
     c->slot_seq.release(); c->slot_w.release();
     c->slot_len.release(); c->slot_col.release(); c->groups_d.release(); c->cand.release();
     c->counters.release();

@@ -34,6 +34,9 @@ constexpr int kBf16Tile = 32;        // windows per bf16 tile (MFMA M)
 constexpr int kFp32Cols = 16;        // columns per fp32 column block
 constexpr int kFp32Tile = 16;        // windows per fp32 tile
 
+// Batch slot without a column.
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
 // Counter slots in device memory.
 enum { kCntCand = 0, kCntHits = 1, kCntBand = 2, kCntQueue = 4, kCntSlots = 12 };
 
@@ -67,13 +70,12 @@ struct FilterArgs {
     const uint64_t* lcmask;      // lowercase bits (MASK variants only)
     const void* frag_a;          // bf16: hi fragments; fp32: weights
     const void* frag_b;          // bf16: lo fragments
-    const float* thr;            // per slot: threshold; list mode: request r at thr + r * req_stride
+    const float* thr;            // per slot: threshold
     const double* band;          // per slot: filter band (MFMA score vs FP64 score)
     const uint32_t* groups;      // column groups, see group_pack()
-    uint2* cand;                 // (window, slot | request << 16 in list mode)
+    uint2* cand;                 // (first window of a lane, slot)
     unsigned long long* counters;
     uint64_t cand_cap;
-    uint32_t req_stride;         // list mode: floats between two requests' tables
 };
 
 // Largest float <= t - b and smallest float >= t + b (b >= 0), for the
@@ -242,8 +244,7 @@ __device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const uint4* o
 // 8(r/4) + 4(l/32) + r%4, i.e. windows qlane + {0..3, 8..11, 16..19, 24..27}.
 template <int NB>
 __device__ __forceinline__ void bf16_epilogue(const FilterArgs& a, Stage& st, const f32x16 (&acc)[NB],
-                                              const float (&th)[NB], uint32_t q0, uint32_t blk0, uint32_t rtag,
-                                              uint32_t lane)
+                                              const float (&th)[NB], uint32_t q0, uint32_t blk0, uint32_t lane)
 {
     const uint32_t qlane = q0 + 4 * (lane >> 5);
 #pragma unroll
@@ -251,7 +252,7 @@ __device__ __forceinline__ void bf16_epilogue(const FilterArgs& a, Stage& st, co
         float mx = acc[b][0];
 #pragma unroll
         for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[b][r]);
-        emit_lane(a, st, mx >= th[b], qlane, ((blk0 + b) * kBf16Cols + (lane & 31)) | rtag);
+        emit_lane(a, st, mx >= th[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
     }
 }
 
@@ -268,16 +269,14 @@ __device__ __forceinline__ uint4 onehot_entry(uint32_t e)
     return make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
 }
 
-// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1, filter thresholds thr
-// (per slot), records tagged with rtag.  The tile loop is
+// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1.  The tile loop is
 // software-pipelined over two accumulator sets: the MFMAs of tile t are
 // issued before the threshold test of tile t-1, so the epilogue's VALU work
 // and its branch overlap matrix-core work of the same wave.  Packed words are
 // read two tiles ahead.
 template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const uint4* onehot, uint32_t blk0,
-                                           const float* __restrict__ thr, uint32_t rtag, uint32_t t0, uint32_t t1,
-                                           uint32_t lane)
+                                           uint32_t t0, uint32_t t1, uint32_t lane)
 {
     const uint4* __restrict__ fh = (const uint4*)a.frag_a;
     const uint4* __restrict__ fl = (const uint4*)a.frag_b;
@@ -293,7 +292,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
             bl[b][j] = __builtin_bit_cast(bf16x8, l);
         }
         const uint32_t slot = (blk0 + b) * kBf16Cols + (lane & 31);
-        th[b] = thr_minus(thr[slot], a.band[slot]);
+        th[b] = thr_minus(a.thr[slot], a.band[slot]);
     }
 
     const uint32_t off = (lane & 31) + 2 * (lane >> 5);   // first base of the lane's window, 0..33
@@ -317,7 +316,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
             bf16_mfma_tile<NCH, NB, MASK>(acc, onehot, bh, bl, funnel64(c0, c1, sh, sh_inv),
                                           MASK ? funnel32(cm0, cm1, off & 31) : 0u);
             mfma_result_guard();
-            bf16_epilogue<NB>(a, st, acc, th, t * kBf16Tile, blk0, rtag, lane);
+            bf16_epilogue<NB>(a, st, acc, th, t * kBf16Tile, blk0, lane);
             c0 = c1; c1 = c2;
             if (MASK) { cm0 = cm1; cm1 = cm2; }
         }
@@ -344,24 +343,24 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
         if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
         bf16_mfma_tile<NCH, NB, MASK>(accB, onehot, bh, bl, funnel64(w0, w1, sh, sh_inv),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
-        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, rtag, lane);
+        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
         bf16_mfma_tile<NCH, NB, MASK>(accA, onehot, bh, bl, funnel64(w1, w2, sh, sh_inv),
                                       MASK ? funnel32(m1, m2, off & 31) : 0u);
-        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, rtag, lane);
+        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
         w0 = w2; w1 = w3; w2 = w4;
         if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
     if (t < t1) {                   // one tile left: t
         bf16_mfma_tile<NCH, NB, MASK>(accB, onehot, bh, bl, funnel64(w0, w1, sh, sh_inv),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
-        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, rtag, lane);
+        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
         pendingA = false;
         mfma_result_guard();
-        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, rtag, lane);
+        bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
     }
     if (pendingA) {
         mfma_result_guard();
-        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, rtag, lane);
+        bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
     }
 }
 
@@ -375,7 +374,7 @@ template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, const float (&bw)[NB][4 * NCH],
                                              const float (&th)[NB], uint32_t q0,
                                              uint64_t bits, uint32_t mb, uint32_t k, uint32_t blk0,
-                                             uint32_t rtag, uint32_t lane)
+                                             uint32_t lane)
 {
     constexpr int NP = 4 * NCH;
     f32x4 acc[NB];
@@ -396,14 +395,13 @@ __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, con
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const float mx = fmaxf(fmaxf(acc[b][0], acc[b][1]), fmaxf(acc[b][2], acc[b][3]));
-        emit_lane(a, st, mx >= th[b], qlane, ((blk0 + b) * kFp32Cols + (lane & 15)) | rtag);
+        emit_lane(a, st, mx >= th[b], qlane, (blk0 + b) * kFp32Cols + (lane & 15));
     }
 }
 
 // Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1 (FP32 path).
 template <int NCH, int NB, bool MASK>
-__device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint32_t blk0,
-                                           const float* __restrict__ thr, uint32_t rtag, uint32_t t0, uint32_t t1,
+__device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint32_t blk0, uint32_t t0, uint32_t t1,
                                            uint32_t lane)
 {
     const float* __restrict__ fw = (const float*)a.frag_a;
@@ -415,7 +413,7 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
 #pragma unroll
         for (int p = 0; p < NP; ++p) bw[b][p] = fw[((size_t)(blk0 + b) * kMaxLen + p) * 64 + lane];
         const uint32_t slot = (blk0 + b) * kFp32Cols + (lane & 15);
-        th[b] = thr_minus(thr[slot], a.band[slot]);
+        th[b] = thr_minus(a.thr[slot], a.band[slot]);
     }
     const uint32_t i = lane & 15, k = lane >> 4;
     const uint32_t s0 = 2 * i, s1 = 2 * (i + 16);    // bit offsets of the two sub-tiles
@@ -446,8 +444,8 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
                 const uint64_t b1 = funnel64(w[u], w[u + 1], s1, 63 - s1);
                 const uint32_t m0 = MASK ? funnel32(mw[u], mw[u + 1], i) : 0u;
                 const uint32_t m1 = MASK ? funnel32(mw[u], mw[u + 1], i + 16) : 0u;
-                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0, b0, m0, k, blk0, rtag, lane);
-                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0 + kFp32Tile, b1, m1, k, blk0, rtag, lane);
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0, b0, m0, k, blk0, lane);
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0 + kFp32Tile, b1, m1, k, blk0, lane);
             }
         }
         w[0] = w[kGroup];
@@ -472,7 +470,9 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
 //     (j / ngroups) * kXcds + x, group j % ngroups;
 //   list form (a batch of requests): sub-queue x holds
 //     items[sub_begin[x] .. sub_begin[x+1]), each (chunk, group), and a chunk
-//     is (first tile, end tile, request).
+//     is (first tile, end tile, request); groups, fragments and per-slot
+//     tables then describe the batch's own column blocks (each request's
+//     selected columns packed into blocks of their own).
 constexpr int kXcds = 8;
 struct WorkQueue {
     uint32_t ntiles;
@@ -489,7 +489,8 @@ struct BatchDesc {
     uint32_t nreq;
     uint32_t nbases;
     uint32_t nruns;
-    uint32_t pad[4];
+    uint32_t nblk;          // column blocks of the batch
+    uint32_t pad[3];
 };
 
 struct ListArgs {
@@ -501,13 +502,12 @@ struct ListArgs {
 // One item: tiles [t0, t1) against one column group.
 template <bool BF, bool MASK>
 __device__ __forceinline__ void filter_item(const FilterArgs& a, Stage& st, const uint4* onehot, uint32_t grp,
-                                            const float* thr, uint32_t rtag, uint32_t t0, uint32_t t1,
-                                            uint32_t lane)
+                                            uint32_t t0, uint32_t t1, uint32_t lane)
 {
     const uint32_t blk0 = grp & 0xFFFFFu, nch = (grp >> 20) & 63u, nb = grp >> 26;
     if constexpr (BF) {
         switch (nch * 8 + nb) {
-#define BLAMM_BF(N, B) case N * 8 + B: bf16_range<N, B, MASK>(a, st, onehot, blk0, thr, rtag, t0, t1, lane); break;
+#define BLAMM_BF(N, B) case N * 8 + B: bf16_range<N, B, MASK>(a, st, onehot, blk0, t0, t1, lane); break;
         BLAMM_BF(1, 1) BLAMM_BF(2, 1) BLAMM_BF(3, 1) BLAMM_BF(4, 1)
         BLAMM_BF(5, 1) BLAMM_BF(6, 1) BLAMM_BF(7, 1) BLAMM_BF(8, 1)
         BLAMM_BF(1, 2) BLAMM_BF(2, 2) BLAMM_BF(3, 2) BLAMM_BF(4, 2)
@@ -518,7 +518,7 @@ __device__ __forceinline__ void filter_item(const FilterArgs& a, Stage& st, cons
     } else {
         (void)onehot;
         switch (nch * 8 + nb) {
-#define BLAMM_F32(N, B) case N * 8 + B: fp32_range<N, B, MASK>(a, st, blk0, thr, rtag, t0, t1, lane); break;
+#define BLAMM_F32(N, B) case N * 8 + B: fp32_range<N, B, MASK>(a, st, blk0, t0, t1, lane); break;
         BLAMM_F32(1, 1) BLAMM_F32(2, 1) BLAMM_F32(3, 1) BLAMM_F32(4, 1)
         BLAMM_F32(5, 1) BLAMM_F32(6, 1) BLAMM_F32(7, 1) BLAMM_F32(8, 1)
         BLAMM_F32(1, 2) BLAMM_F32(2, 2) BLAMM_F32(3, 2) BLAMM_F32(4, 2)
@@ -558,24 +558,20 @@ __global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs 
             if (lane == 0) j = atomicAdd((unsigned int*)&a.counters[kCntQueue + x], 1u);
             j = __builtin_amdgcn_readfirstlane(__shfl(j, 0));
             if (j >= items) break;
-            uint32_t grp, t0, t1, rtag = 0;
-            const float* thr = a.thr;
+            uint32_t grp, t0, t1;
             if constexpr (LIST) {
                 const uint2 it = l.items[base + j];
                 const uint4 ch = l.chunks[__builtin_amdgcn_readfirstlane(it.x)];
                 grp = __builtin_amdgcn_readfirstlane(a.groups[__builtin_amdgcn_readfirstlane(it.y)]);
                 t0 = __builtin_amdgcn_readfirstlane(ch.x);
                 t1 = __builtin_amdgcn_readfirstlane(ch.y);
-                const uint32_t req = __builtin_amdgcn_readfirstlane(ch.z);
-                rtag = req << 16;
-                thr = a.thr + (size_t)req * a.req_stride;
             } else {
                 const uint32_t k = (j / q.ngroups) * kXcds + x;
                 grp = __builtin_amdgcn_readfirstlane(a.groups[j % q.ngroups]);
                 t0 = k * q.chunk_tiles;
                 t1 = min(q.ntiles, t0 + q.chunk_tiles);
             }
-            filter_item<BF, MASK>(a, st, onehot, grp, thr, rtag, t0, t1, lane);
+            filter_item<BF, MASK>(a, st, onehot, grp, t0, t1, lane);
         }
     }
     stage_flush(a, st);
@@ -593,14 +589,15 @@ struct FinalizeArgs {
     uint32_t nruns;
     const int* slot_len;
     const float* slot_w;                  // [slot][kMaxLen][4] FP32 weights
-    const float* slot_thr;                // the threshold; list form: per request (req_stride)
+    const float* slot_thr;                // the threshold per slot (list form: per batch slot)
     const double* slot_seq;               // sequential-FP32 band per slot
     const uint32_t* slot_col;             // caller's column index
     void* hits;                           // blamm_hit[]
     uint64_t hit_cap;
-    uint32_t req_stride;                  // list form: floats between two requests' tables
     const BatchDesc* desc;                // list form: nbases, nruns from here
     const uint32_t* req_end;              // list form: end of each request's bases
+    const uint32_t* slot_map;             // list form: batch slot -> library slot
+    const uint32_t* blk_req;              // list form: request of each batch block
 };
 
 struct DevHit {
@@ -656,21 +653,24 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
     };
     for (uint64_t base = wave * 64; base < ncand; base += nw * 64) {
         const uint64_t idx = base + lane;
-        const bool have = idx < ncand;
-        uint32_t q0 = 0, slot = 0, req = 0, qend = nbases;
+        bool have = idx < ncand;
+        // slot: threshold slot of the record; lib: library slot holding the
+        // column's weights (the same slot outside the list form)
+        uint32_t q0 = 0, slot = 0, lib = 0, req = 0, qend = nbases;
         if (have) {
             const uint2 cd = a.cand[idx];
             q0 = cd.x;
-            slot = cd.y;
+            slot = lib = cd.y;
             if constexpr (LIST) {
-                slot = cd.y & 0xFFFFu;
-                req = cd.y >> 16;
+                constexpr uint32_t cpb = NR == 16 ? kBf16Cols : kFp32Cols;
+                req = a.blk_req[slot / cpb];
+                lib = a.slot_map[slot];
                 qend = min(nbases, a.req_end[req]);
+                if (lib == kNoSlot) have = false, lib = 0;   // empty slots never pass the filter
             }
         }
-        const size_t toff = LIST ? (size_t)req * a.req_stride + slot : (size_t)slot;
-        const int L = have ? a.slot_len[slot] : 0;
-        const float* w = a.slot_w + (size_t)slot * kMaxLen * 4;
+        const int L = have ? a.slot_len[lib] : 0;
+        const float* w = a.slot_w + (size_t)lib * kMaxLen * 4;
         // packed bases q0 .. q0+63 and their lowercase bits
         uint64_t lo = 0, hi = 0, mlo = 0;
         uint32_t nb1 = 0;
@@ -702,11 +702,11 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             wh >>= 2;
             ml >>= 1;
         }
-        const float thr = have ? a.slot_thr[toff] : 0.f;
-        const double seq = have ? a.slot_seq[slot] : 0.0;
+        const float thr = have ? a.slot_thr[slot] : 0.f;
+        const double seq = have ? a.slot_seq[lib] : 0.0;
         const float shi = thr_plus(thr, seq);
         const float slo = thr_minus(thr, seq);
-        const uint32_t col = have ? a.slot_col[slot] : 0u;
+        const uint32_t col = have ? a.slot_col[lib] : 0u;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             const uint32_t q = q0 + record_offset<NR>(r);
@@ -888,5 +888,60 @@ __global__ __launch_bounds__(kGroupThreads) void group_scatter_kernel(GroupArgs 
     }
 }
 static_assert(kCntQueue + kXcds <= kCntSlots, "queue counters");
+
+// Column blocks of a batch: batch slot s takes the fragments of library slot
+// slot_map[s] (an empty slot gets zero weights).  Bands are the library's
+// bands for the longest block (valid for a column in any block).
+struct GatherArgs {
+    const BatchDesc* desc;                // nblk
+    const uint32_t* slot_map;
+    const uint4* lib_a;                   // bf16: [block][chunk][lane] hi / fp32: weights as uint4 view unused
+    const uint4* lib_b;
+    const float* lib_w;                   // fp32: [block][position][lane]
+    const double* lib_band;               // per library slot, any block
+    uint4* out_a;
+    uint4* out_b;
+    float* out_w;
+    double* out_band;
+};
+
+template <bool BF>
+__global__ __launch_bounds__(256) void gather_kernel(GatherArgs g)
+{
+    const uint32_t nblk = g.desc->nblk;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    const uint64_t t0 = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    constexpr uint32_t cpb = BF ? kBf16Cols : kFp32Cols;
+    if constexpr (BF) {
+        const uint64_t total = (uint64_t)nblk * kMaxChunks * 64;
+        for (uint64_t i = t0; i < total; i += stride) {
+            const uint32_t l = (uint32_t)(i & 63), j = (uint32_t)((i >> 6) % kMaxChunks);
+            const uint32_t b = (uint32_t)(i / (64 * kMaxChunks));
+            const uint32_t lib = g.slot_map[b * cpb + (l & 31)];
+            uint4 va = make_uint4(0, 0, 0, 0), vb = va;
+            if (lib != kNoSlot) {
+                const size_t src = ((size_t)(lib / cpb) * kMaxChunks + j) * 64 + (lib % cpb) + 32 * (l >> 5);
+                va = g.lib_a[src];
+                vb = g.lib_b[src];
+            }
+            g.out_a[i] = va;
+            g.out_b[i] = vb;
+        }
+    } else {
+        const uint64_t total = (uint64_t)nblk * kMaxLen * 64;
+        for (uint64_t i = t0; i < total; i += stride) {
+            const uint32_t l = (uint32_t)(i & 63), p = (uint32_t)((i >> 6) % kMaxLen);
+            const uint32_t b = (uint32_t)(i / (64 * kMaxLen));
+            const uint32_t lib = g.slot_map[b * cpb + (l & 15)];
+            float v = 0.f;
+            if (lib != kNoSlot) v = g.lib_w[((size_t)(lib / cpb) * kMaxLen + p) * 64 + (lib % cpb) + 16 * (l >> 4)];
+            g.out_w[i] = v;
+        }
+    }
+    for (uint64_t s = t0; s < (uint64_t)nblk * cpb; s += stride) {
+        const uint32_t lib = g.slot_map[s];
+        g.out_band[s] = lib != kNoSlot ? g.lib_band[lib] : 0.0;
+    }
+}
 
 }  // namespace blamm

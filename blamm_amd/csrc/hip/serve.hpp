@@ -121,11 +121,16 @@ void serve_loop(blamm_hip_server* S)
             if (!oldest || blamm_hip_batch_query(oldest->batch) == 1) break;
             lk.unlock();
             uint64_t nh = 0;
+            const uint64_t t0 = now_ns();
+            const uint64_t rp0 = oldest->batch->replays;
             const int rc = blamm_hip_batch_wait(oldest->batch, &nh);
             blamm_scan_stats bs{};
             blamm_hip_batch_stats(oldest->batch, &bs);
+            const uint64_t t1 = now_ns();
             lk.lock();
             S->stats.device_ms += bs.kernel_ms;
+            S->stats.host_wait_ms += (t1 - t0) / 1e6;
+            S->stats.replays += oldest->batch->replays - rp0;
             serve_finish(S, *oldest, rc, lk);
             progressed = true;
         }
@@ -157,7 +162,9 @@ void serve_loop(blamm_hip_server* S)
             S->stats.batches++;
             lk.unlock();
             const int rc = blamm_hip_batch_submit(sl.batch, batch.data(), (uint32_t)batch.size());
+            const uint64_t t1 = now_ns();
             lk.lock();
+            S->stats.host_submit_ms += (t1 - t) / 1e6;
             if (rc != BLAMM_OK) serve_finish(S, sl, rc, lk);
             progressed = true;
         }

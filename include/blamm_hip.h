@@ -215,7 +215,9 @@ int blamm_hip_batch_destroy(blamm_hip_batch *batch);
 /* Device bytes the slot holds now. */
 uint64_t blamm_hip_batch_device_bytes(const blamm_hip_batch *batch);
 /* Stage nreq (< 65536) requests and launch them (asynchronous).  Request
- * buffers may be reused as soon as this returns. */
+ * buffers may be reused as soon as this returns.  Scratch and hit buffers are
+ * grown up front to the requests' expected_hits (when given), so a batch is
+ * rarely run twice. */
 int blamm_hip_batch_submit(blamm_hip_batch *batch, const blamm_request *reqs, uint32_t nreq);
 /* 0 when the batch has finished on the device (or none is in flight),
  * 1 while it runs. */
@@ -249,7 +251,10 @@ typedef struct blamm_request_info {
 
 typedef struct blamm_server_stats {
     uint64_t submitted, completed, batches, hits;
+    uint64_t replays;             /* batches run again after growing scratch */
     double device_ms;             /* summed batch launch times */
+    double host_submit_ms;        /* worker time staging and launching batches */
+    double host_wait_ms;          /* worker time in waits and hit read-back */
 } blamm_server_stats;
 
 typedef struct blamm_hip_server blamm_hip_server;
