@@ -551,7 +551,7 @@ def main():
                    "bases": nbp, "columns": ncols, "pvalue": args.pvalue, "precision": "split-bf16"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
-                     "kernel": "filter_persistent_kernel<true,2,false> (one launch per step)",
+                     "kernel": "filter_persistent_kernel<true,false,false> (one launch per step)",
                      "algorithmic_flops": flops},
         "hbm": {"algorithmic_bytes": alg_bytes, "achieved_GBs": alg_bytes / (bf["kernel_ms"] / 1e3) / 1e9,
                 "peak_GBs": PEAK_HBM_GBS},
