@@ -20,7 +20,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 BIN_DIR = os.path.join(PKG_DIR, "bin")
-HIP_LIB = os.path.join(LIB_DIR, "libblamm_hip.so")
+HIP_LIB = os.environ.get("BLAMM_HIP_LIB") or os.path.join(LIB_DIR, "libblamm_hip.so")
 HOST_LIB = os.path.join(LIB_DIR, "libblamm_host.so")
 BLAMM_BIN = os.path.join(BIN_DIR, "blamm")
 

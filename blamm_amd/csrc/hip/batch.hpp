@@ -83,7 +83,7 @@ struct blamm_hip_batch {
     DevBuf<uint32_t> blk_req;       // per batch block: request
     DevBuf<uint32_t> groups;        // column groups (group_pack)
     DevBuf<uint4> frag_a, frag_b;   // gathered fragments (fp32 weights in frag_a)
-    DevBuf<double> band;            // per batch slot
+    DevBuf<double> band, band_hi;   // per batch slot
     DevBuf<uint32_t> req_start, req_end, req_count;
     DevBuf<unsigned long long> req_off, cursor;
     DevBuf<uint2> items, cand;
@@ -149,10 +149,12 @@ int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
     ga.lib_b = (const uint4*)c->frag_b.p;
     ga.lib_w = (const float*)c->frag_a.p;
     ga.lib_band = c->slot_band_any.p;
+    ga.lib_band_hi = c->slot_band_hi_any.p;
     ga.out_a = b->frag_a.p;
     ga.out_b = b->frag_b.p;
     ga.out_w = (float*)b->frag_a.p;
     ga.out_band = b->band.p;
+    ga.out_band_hi = b->band_hi.p;
     if (bf) hipLaunchKernelGGL(gather_kernel<true>, dim3((uint32_t)c->num_cu * 4), dim3(256), 0, b->stream, ga);
     else hipLaunchKernelGGL(gather_kernel<false>, dim3((uint32_t)c->num_cu * 4), dim3(256), 0, b->stream, ga);
     HIP_TRY(hipGetLastError());
@@ -163,6 +165,7 @@ int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
     a.frag_b = b->frag_b.p;
     a.thr = b->thr.p;
     a.band = b->band.p;
+    a.band_hi = b->band_hi.p;
     a.groups = b->groups.p;
     a.cand = b->cand.p;
     a.counters = b->counters.p;
@@ -284,6 +287,7 @@ int blamm_hip_batch_destroy(blamm_hip_batch* b)
     drop_graphs(b);
     b->packed.release(); b->lcmask.release(); b->runs.release(); b->thr.release(); b->slot_map.release();
     b->blk_req.release(); b->groups.release(); b->frag_a.release(); b->frag_b.release(); b->band.release();
+    b->band_hi.release();
     b->req_start.release();
     b->req_end.release(); b->req_count.release(); b->req_off.release(); b->cursor.release();
     b->items.release(); b->cand.release(); b->chunks.release(); b->desc.release(); b->counters.release();
@@ -302,7 +306,7 @@ uint64_t blamm_hip_batch_device_bytes(const blamm_hip_batch* b)
 {
     if (!b) return 0;
     return b->packed.n * 8 + b->lcmask.n * 8 + b->runs.n * 8 + b->thr.n * 4 + b->slot_map.n * 4 + b->blk_req.n * 4 +
-           b->groups.n * 4 + (b->frag_a.n + b->frag_b.n) * 16 + b->band.n * 8 +
+           b->groups.n * 4 + (b->frag_a.n + b->frag_b.n) * 16 + (b->band.n + b->band_hi.n) * 8 +
            (b->req_start.n + b->req_end.n + b->req_count.n) * 4 + (b->req_off.n + b->cursor.n) * 8 + b->items.n * 8 +
            b->cand.n * 8 + b->chunks.n * 16 + (b->hits.n + b->grouped.n) * sizeof(blamm_hit) + sizeof(BatchDesc) +
            kCntSlots * 8;
@@ -438,6 +442,7 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     BLAMM_TRY(batch_grow(b, b->thr, nbs));
     BLAMM_TRY(batch_grow(b, b->slot_map, nbs));
     BLAMM_TRY(batch_grow(b, b->band, nbs));
+    BLAMM_TRY(batch_grow(b, b->band_hi, nbs));
     BLAMM_TRY(batch_grow(b, b->blk_req, std::max<size_t>(nblk, 1)));
     BLAMM_TRY(batch_grow(b, b->groups, std::max<size_t>(gpk.size(), 1)));
     const size_t fr = (size_t)std::max<uint32_t>(nblk, 1) * (bf ? (size_t)kMaxChunks * 64 : (size_t)kMaxLen * 16);

@@ -147,6 +147,7 @@ struct blamm_hip_ctx {
     DevBuf<uint8_t> frag_a, frag_b;
     DevBuf<float> slot_thr, slot_w;
     DevBuf<double> slot_band, slot_band_any, slot_seq;   // band_any: valid in a block of any length
+    DevBuf<double> slot_band_hi, slot_band_hi_any;      // hi-only stage of the bf16 filter
     DevBuf<int> slot_len;
     DevBuf<uint32_t> slot_col, groups_d;
 
@@ -292,6 +293,7 @@ int relayout(blamm_hip_ctx* c)
     std::vector<int> s_len(nslots, 1), s_colh(nslots, -1);
     std::vector<uint32_t> s_col(nslots, 0);
     std::vector<double> s_band(nslots, 0.0), s_band_any(nslots, 0.0), s_seq(nslots, 0.0);
+    std::vector<double> s_band_hi(nslots, 0.0), s_band_hi_any(nslots, 0.0);
     std::vector<uint16_t> hi16((size_t)nslots * kMaxLen * 4, 0), lo16((size_t)nslots * kMaxLen * 4, 0);
     double max_band = 0.0, max_err = 0.0;
     const double u24 = std::ldexp(1.0, -24), u53 = std::ldexp(1.0, -53);
@@ -303,9 +305,9 @@ int relayout(blamm_hip_ctx* c)
             const int slot = bi * cols_per_blk + k;
             const int L = lengths[col];
             const float* w = P + (size_t)col * ldp;
-            double sum_max = 0.0, sum_split = 0.0, rep_err = 0.0;
+            double sum_max = 0.0, sum_split = 0.0, rep_err = 0.0, sum_lo = 0.0;
             for (int p = 0; p < L; ++p) {
-                double mx = 0.0, mxs = 0.0, me = 0.0;
+                double mx = 0.0, mxs = 0.0, me = 0.0, ml = 0.0;
                 for (int b = 0; b < 4; ++b) {
                     const float x = w[4 * p + b];
                     s_w[((size_t)slot * kMaxLen + p) * 4 + b] = x;
@@ -319,6 +321,7 @@ int relayout(blamm_hip_ctx* c)
                         lo16[((size_t)slot * kMaxLen + p) * 4 + b] = l;
                         me = std::max(me, std::fabs((double)x - (double)hf - (double)lf));
                         mxs = std::max(mxs, std::fabs((double)hf) + std::fabs((double)lf));
+                        ml = std::max(ml, std::fabs((double)lf));
                     } else {
                         mxs = std::max(mxs, (double)std::fabs(x));
                     }
@@ -326,6 +329,7 @@ int relayout(blamm_hip_ctx* c)
                 sum_max += mx;
                 sum_split += mxs;
                 rep_err += me;
+                sum_lo += ml;
             }
             // |FP64 left-to-right sum - exact sum| (binary64, L-1 additions)
             const double fp64_err = gamma_n(L, u53) * sum_max;
@@ -338,6 +342,12 @@ int relayout(blamm_hip_ctx* c)
             const double nterms_any = bf ? 32.0 * kMaxChunks : 16.0 * kMaxChunks;
             s_band_any[slot] =
                 2.0 * (rep_err + gamma_n(nterms_any, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
+            // hi-only score H': the lo terms it leaves out are bounded by
+            // sum_p max_b |lo|; rounding as for the full sum
+            s_band_hi[slot] =
+                2.0 * (sum_lo + rep_err + gamma_n(nterms, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
+            s_band_hi_any[slot] =
+                2.0 * (sum_lo + rep_err + gamma_n(nterms_any, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
             // Sequential FP32 band used by finalize (RNE adds, L-1 of them).
             const double seq = 2.0 * (gamma_n(L, u24) * sum_max + fp64_err) + 1e-30;
             s_thr[slot] = c->lib_thr[col];
@@ -394,6 +404,8 @@ int relayout(blamm_hip_ctx* c)
     HIP_TRY(c->slot_thr.reserve(nslots));
     HIP_TRY(c->slot_band.reserve(nslots));
     HIP_TRY(c->slot_band_any.reserve(nslots));
+    HIP_TRY(c->slot_band_hi.reserve(nslots));
+    HIP_TRY(c->slot_band_hi_any.reserve(nslots));
     HIP_TRY(c->slot_seq.reserve(nslots));
     HIP_TRY(c->slot_w.reserve(s_w.size()));
     HIP_TRY(c->slot_len.reserve(nslots));
@@ -404,6 +416,9 @@ int relayout(blamm_hip_ctx* c)
     HIP_TRY(hipMemcpy(c->slot_thr.p, s_thr.data(), nslots * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_band.p, s_band.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_band_any.p, s_band_any.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_band_hi.p, s_band_hi.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->slot_band_hi_any.p, s_band_hi_any.data(), nslots * sizeof(double),
+                      hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_seq.p, s_seq.data(), nslots * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_w.p, s_w.data(), s_w.size() * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->slot_len.p, s_len.data(), nslots * sizeof(int), hipMemcpyHostToDevice));
@@ -432,6 +447,7 @@ FilterArgs filter_args(const blamm_hip_ctx* c, const uint64_t* packed, const uin
     a.frag_b = c->frag_b.p;
     a.thr = thr;
     a.band = c->slot_band.p;
+    a.band_hi = c->slot_band_hi.p;
     a.groups = c->groups_d.p;
     a.cand = cand;
     a.counters = counters;
@@ -531,7 +547,7 @@ int blamm_hip_destroy(blamm_hip_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->frag_a.release(); c->frag_b.release(); c->slot_thr.release(); c->slot_band.release();
-    c->slot_band_any.release();
+    c->slot_band_any.release(); c->slot_band_hi.release(); c->slot_band_hi_any.release();
     c->slot_seq.release(); c->slot_w.release();
     c->slot_len.release(); c->slot_col.release(); c->groups_d.release(); c->cand.release();
     c->counters.release();

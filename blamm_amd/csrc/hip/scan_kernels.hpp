@@ -72,6 +72,7 @@ struct FilterArgs {
     const void* frag_b;          // bf16: lo fragments
     const float* thr;            // per slot: threshold
     const double* band;          // per slot: filter band (MFMA score vs FP64 score)
+    const double* band_hi;       // per slot: band of the hi-only score (bf16 lazy-lo stage)
     const uint32_t* groups;      // column groups, see group_pack()
     uint2* cand;                 // (first window of a lane, slot)
     unsigned long long* counters;
@@ -176,20 +177,16 @@ __device__ __forceinline__ uint32_t funnel32(uint32_t lo, uint32_t hi, uint32_t 
 }
 
 // Tiles are 32 consecutive windows; a wave walks tiles [t0, t1) of its
-// range.  Lane-private bases for tile t come from packed words t+woff and
-// t+woff+1 (a sliding pair), so the loop keeps a register window of kGroup+1
-// words and prefetches the next group's words one group ahead.
+// range.  The FP32 loop keeps a register window of kGroup+1 packed words and
+// prefetches the next group's words one group ahead.
 constexpr int kGroup = 4;
 
-// Tile loop of the bf16 filter: 1 = two accumulator sets (software pipeline),
-// 0 = one set (fewer registers, more resident waves).
-#ifndef BLAMM_PIPELINE
-#define BLAMM_PIPELINE 1
-#endif
-
-// One-hot operand source of the bf16 filter: 1 = LDS table, 0 = VALU.
-#ifndef BLAMM_ONEHOT_LDS
-#define BLAMM_ONEHOT_LDS 0
+// Lo products of the bf16 filter: 1 = only for tile blocks whose hi-only
+// score can still reach the threshold (one accumulator set per wave), 0 =
+// always (two accumulator sets, software-pipelined).  1 is about 30% faster
+// at p = 1e-4.
+#ifndef BLAMM_LAZY_LO
+#define BLAMM_LAZY_LO 1
 #endif
 
 // ---------------------------------------------------------------------------
@@ -201,37 +198,33 @@ constexpr int kGroup = 4;
 // Each wave keeps NB column blocks (NCH chunks, hi and lo) in registers and
 // streams window tiles through them.
 // ---------------------------------------------------------------------------
+// The one-hot operand of chunk j for the lane's window: bf16 1.0 (0x3F80) in
+// lane b of four per base, built in registers (no memory latency; the VALU
+// work overlaps earlier MFMAs); a lowercase base gets an all-zero row.
+template <bool MASK>
+__device__ __forceinline__ bf16x8 onehot_chunk(uint64_t bits, uint32_t mb, int j)
+{
+    const uint32_t b0 = (uint32_t)(bits >> (8 * j)) & 3u;
+    const uint32_t b1 = (uint32_t)(bits >> (8 * j + 2)) & 3u;
+    uint64_t o0 = 0x3F80ull << (b0 * 16);
+    uint64_t o1 = 0x3F80ull << (b1 * 16);
+    if (MASK) {
+        if ((mb >> (4 * j)) & 1) o0 = 0;
+        if ((mb >> (4 * j + 1)) & 1) o1 = 0;
+    }
+    const u32x4 av = {(uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32)};
+    return __builtin_bit_cast(bf16x8, av);
+}
+
 template <int NCH, int NB, bool MASK>
-__device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const uint4* onehot,
-                                               const bf16x8 (&bh)[NB][NCH], const bf16x8 (&bl)[NB][NCH],
-                                               uint64_t bits, uint32_t mb)
+__device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const bf16x8 (&bh)[NB][NCH],
+                                               const bf16x8 (&bl)[NB][NCH], uint64_t bits, uint32_t mb)
 {
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
 #pragma unroll
     for (int j = 0; j < NCH; ++j) {
-#if BLAMM_ONEHOT_LDS
-        // the lane's two bases of chunk j (and their lowercase bits) index the
-        // one-hot table in LDS
-        uint32_t e = (uint32_t)(bits >> (8 * j)) & 15u;
-        if (MASK) e |= ((mb >> (4 * j)) & 3u) << 4;
-        const bf16x8 A = __builtin_bit_cast(bf16x8, onehot[e]);
-#else
-        // built in registers: bf16 1.0 (0x3F80) in lane b of four per base;
-        // no memory latency, and the VALU work overlaps the previous chunk's
-        // MFMAs
-        (void)onehot;
-        const uint32_t b0 = (uint32_t)(bits >> (8 * j)) & 3u;
-        const uint32_t b1 = (uint32_t)(bits >> (8 * j + 2)) & 3u;
-        uint64_t o0 = 0x3F80ull << (b0 * 16);
-        uint64_t o1 = 0x3F80ull << (b1 * 16);
-        if (MASK) {
-            if ((mb >> (4 * j)) & 1) o0 = 0;
-            if ((mb >> (4 * j + 1)) & 1) o1 = 0;
-        }
-        const u32x4 av = {(uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32)};
-        const bf16x8 A = __builtin_bit_cast(bf16x8, av);
-#endif
+        const bf16x8 A = onehot_chunk<MASK>(bits, mb, j);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bh[b][j], acc[b], 0, 0, 0);
@@ -256,32 +249,75 @@ __device__ __forceinline__ void bf16_epilogue(const FilterArgs& a, Stage& st, co
     }
 }
 
-// One-hot bf16 fragments for two bases (and two lowercase bits): entry
-// e = b0 | b1 << 2 | m0 << 4 | m1 << 5 holds 1.0 in lane b0 of the first four
-// bf16 and in lane b1 of the last four, zeroed for a lowercase base.
-__device__ __forceinline__ uint4 onehot_entry(uint32_t e)
+// ---------------------------------------------------------------------------
+// Lazy lo.  The hi products alone give a score H'; a window's FP64 score can
+// reach thr only if H' >= thr - band_hi, where band_hi bounds the lo terms,
+// the split's representation error and every rounding (proved per column on
+// the host).  So a tile block whose hi-only maximum stays below that bound in
+// every lane cannot hold a candidate and its lo products are skipped; blocks
+// that pass get their lo products and the exact-band test of the plain form.
+// ---------------------------------------------------------------------------
+template <int NCH, int NB, bool MASK>
+__device__ __forceinline__ void bf16_hi_tile(f32x16 (&acc)[NB], const bf16x8 (&bh)[NB][NCH], uint64_t bits,
+                                             uint32_t mb)
 {
-    const uint32_t b0 = e & 3u, b1 = (e >> 2) & 3u;
-    uint64_t o0 = 0x3F80ull << (b0 * 16);
-    uint64_t o1 = 0x3F80ull << (b1 * 16);
-    if ((e >> 4) & 1u) o0 = 0;
-    if ((e >> 5) & 1u) o1 = 0;
-    return make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
+#pragma unroll
+    for (int j = 0; j < NCH; ++j) {
+        const bf16x8 A = onehot_chunk<MASK>(bits, mb, j);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bh[b][j], acc[b], 0, 0, 0);
+    }
 }
 
-// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1.  The tile loop is
-// software-pipelined over two accumulator sets: the MFMAs of tile t are
-// issued before the threshold test of tile t-1, so the epilogue's VALU work
-// and its branch overlap matrix-core work of the same wave.  Packed words are
-// read two tiles ahead.
+template <int NB>
+__device__ __forceinline__ float tile_max(const f32x16& v)
+{
+    float mx = v[0];
+#pragma unroll
+    for (int r = 1; r < 16; ++r) mx = fmaxf(mx, v[r]);
+    return mx;
+}
+
+// Finish a tile whose hi products are in acc (read no sooner than the MFMAs
+// allow: callers issue other MFMAs or a guard in between).
 template <int NCH, int NB, bool MASK>
-__device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const uint4* onehot, uint32_t blk0,
-                                           uint32_t t0, uint32_t t1, uint32_t lane)
+__device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st, f32x16 (&acc)[NB],
+                                                 const bf16x8 (&bl)[NB][NCH], const float (&th)[NB],
+                                                 const float (&thh)[NB], uint64_t bits, uint32_t mb, uint32_t q0,
+                                                 uint32_t blk0, uint32_t lane)
+{
+    uint32_t need = 0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        if (__ballot(tile_max<NB>(acc[b]) >= thh[b])) need |= 1u << b;
+    if (need == 0) return;
+#pragma unroll
+    for (int j = 0; j < NCH; ++j) {
+        const bf16x8 A = onehot_chunk<MASK>(bits, mb, j);
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (need & (1u << b)) acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bl[b][j], acc[b], 0, 0, 0);
+    }
+    mfma_result_guard();
+    const uint32_t qlane = q0 + 4 * (lane >> 5);
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        if (need & (1u << b))
+            emit_lane(a, st, tile_max<NB>(acc[b]) >= th[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
+}
+
+// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1.  Lane-private bases
+// of tile t come from packed words t+woff and t+woff+1 (a sliding pair).
+template <int NCH, int NB, bool MASK>
+__device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint32_t blk0, uint32_t t0, uint32_t t1,
+                                           uint32_t lane)
 {
     const uint4* __restrict__ fh = (const uint4*)a.frag_a;
     const uint4* __restrict__ fl = (const uint4*)a.frag_b;
     bf16x8 bh[NB][NCH], bl[NB][NCH];
-    float th[NB];
+    float th[NB], thh[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
 #pragma unroll
@@ -293,6 +329,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
         }
         const uint32_t slot = (blk0 + b) * kBf16Cols + (lane & 31);
         th[b] = thr_minus(a.thr[slot], a.band[slot]);
+        thh[b] = thr_minus(a.thr[slot], a.band_hi[slot]);
     }
 
     const uint32_t off = (lane & 31) + 2 * (lane >> 5);   // first base of the lane's window, 0..33
@@ -300,37 +337,36 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
     const uint32_t sh = 2 * (off & 31), sh_inv = 63 - sh;
     const uint64_t* __restrict__ pk = a.packed + woff;
     const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + woff;
-    // rolling words: tile t reads (pk[t], pk[t+1]); on loop entry
-    // (w0, w1, w2) = pk[t..t+2] and pk[t+3], pk[t+4] are fetched for the
-    // next iteration
-    uint32_t m0 = 0, m1 = 0, m2 = 0;
-#if !BLAMM_PIPELINE
-    {
-        // one accumulator set: fewer registers, more resident waves
-        f32x16 acc[NB];
-        uint64_t c0 = pk[t0], c1 = pk[t0 + 1];
-        uint32_t cm0 = MASK ? mk[t0] : 0u, cm1 = MASK ? mk[t0 + 1] : 0u;
-        for (uint32_t t = t0; t < t1; ++t) {
-            const uint64_t c2 = pk[t + 2];
-            const uint32_t cm2 = MASK ? mk[t + 2] : 0u;
-            bf16_mfma_tile<NCH, NB, MASK>(acc, onehot, bh, bl, funnel64(c0, c1, sh, sh_inv),
-                                          MASK ? funnel32(cm0, cm1, off & 31) : 0u);
-            mfma_result_guard();
-            bf16_epilogue<NB>(a, st, acc, th, t * kBf16Tile, blk0, lane);
-            c0 = c1; c1 = c2;
-            if (MASK) { cm0 = cm1; cm1 = cm2; }
-        }
-        (void)m0; (void)m1; (void)m2;
-        return;
+#if BLAMM_LAZY_LO
+    // one accumulator set: hi products of a tile, then its finish (the other
+    // resident wave keeps the matrix cores busy meanwhile)
+    f32x16 acc[NB];
+    uint64_t c0 = pk[t0], c1 = pk[t0 + 1];
+    uint32_t cm0 = MASK ? mk[t0] : 0u, cm1 = MASK ? mk[t0 + 1] : 0u;
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint64_t c2 = pk[t + 2];
+        const uint32_t cm2 = MASK ? mk[t + 2] : 0u;
+        const uint64_t bits = funnel64(c0, c1, sh, sh_inv);
+        const uint32_t mb = MASK ? funnel32(cm0, cm1, off & 31) : 0u;
+        bf16_hi_tile<NCH, NB, MASK>(acc, bh, bits, mb);
+        mfma_result_guard();
+        bf16_finish_tile<NCH, NB, MASK>(a, st, acc, bl, th, thh, bits, mb, t * kBf16Tile, blk0, lane);
+        c0 = c1; c1 = c2;
+        if (MASK) { cm0 = cm1; cm1 = cm2; }
     }
-#endif
+#else
+    // two accumulator sets, software-pipelined: the MFMAs of tile t are
+    // issued before the threshold test of tile t-1; packed words are read
+    // two tiles ahead ((w0, w1, w2) = pk[t..t+2] on loop entry)
+    (void)thh;
     f32x16 accA[NB], accB[NB];
+    uint32_t m0 = 0, m1 = 0, m2 = 0;
     {
         const uint64_t p0 = pk[t0];
         const uint32_t q0m = MASK ? mk[t0] : 0u;
         uint64_t p1 = pk[t0 + 1];
         if (MASK) m0 = mk[t0 + 1];
-        bf16_mfma_tile<NCH, NB, MASK>(accA, onehot, bh, bl, funnel64(p0, p1, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, funnel64(p0, p1, sh, sh_inv),
                                       MASK ? funnel32(q0m, m0, off & 31) : 0u);
     }
     uint64_t w0 = pk[t0 + 1], w1 = pk[t0 + 2], w2 = pk[t0 + 3];
@@ -341,17 +377,17 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
         const uint64_t w3 = pk[t + 3], w4 = pk[t + 4];
         uint32_t m3 = 0, m4 = 0;
         if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
-        bf16_mfma_tile<NCH, NB, MASK>(accB, onehot, bh, bl, funnel64(w0, w1, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, funnel64(w0, w1, sh, sh_inv),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
         bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
-        bf16_mfma_tile<NCH, NB, MASK>(accA, onehot, bh, bl, funnel64(w1, w2, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, funnel64(w1, w2, sh, sh_inv),
                                       MASK ? funnel32(m1, m2, off & 31) : 0u);
         bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
         w0 = w2; w1 = w3; w2 = w4;
         if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
     if (t < t1) {                   // one tile left: t
-        bf16_mfma_tile<NCH, NB, MASK>(accB, onehot, bh, bl, funnel64(w0, w1, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, funnel64(w0, w1, sh, sh_inv),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
         bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
         pendingA = false;
@@ -362,6 +398,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, const
         mfma_result_guard();
         bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
     }
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -501,13 +538,13 @@ struct ListArgs {
 
 // One item: tiles [t0, t1) against one column group.
 template <bool BF, bool MASK>
-__device__ __forceinline__ void filter_item(const FilterArgs& a, Stage& st, const uint4* onehot, uint32_t grp,
+__device__ __forceinline__ void filter_item(const FilterArgs& a, Stage& st, uint32_t grp,
                                             uint32_t t0, uint32_t t1, uint32_t lane)
 {
     const uint32_t blk0 = grp & 0xFFFFFu, nch = (grp >> 20) & 63u, nb = grp >> 26;
     if constexpr (BF) {
         switch (nch * 8 + nb) {
-#define BLAMM_BF(N, B) case N * 8 + B: bf16_range<N, B, MASK>(a, st, onehot, blk0, t0, t1, lane); break;
+#define BLAMM_BF(N, B) case N * 8 + B: bf16_range<N, B, MASK>(a, st, blk0, t0, t1, lane); break;
         BLAMM_BF(1, 1) BLAMM_BF(2, 1) BLAMM_BF(3, 1) BLAMM_BF(4, 1)
         BLAMM_BF(5, 1) BLAMM_BF(6, 1) BLAMM_BF(7, 1) BLAMM_BF(8, 1)
         BLAMM_BF(1, 2) BLAMM_BF(2, 2) BLAMM_BF(3, 2) BLAMM_BF(4, 2)
@@ -516,7 +553,6 @@ __device__ __forceinline__ void filter_item(const FilterArgs& a, Stage& st, cons
         default: break;
         }
     } else {
-        (void)onehot;
         switch (nch * 8 + nb) {
 #define BLAMM_F32(N, B) case N * 8 + B: fp32_range<N, B, MASK>(a, st, blk0, t0, t1, lane); break;
         BLAMM_F32(1, 1) BLAMM_F32(2, 1) BLAMM_F32(3, 1) BLAMM_F32(4, 1)
@@ -536,9 +572,6 @@ __device__ __forceinline__ void filter_item(const FilterArgs& a, Stage& st, cons
 template <bool BF, bool MASK, bool LIST>
 __global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs a, WorkQueue q, ListArgs l)
 {
-    __shared__ uint4 onehot[MASK ? 64 : 16];
-    if (BF && threadIdx.x < (MASK ? 64u : 16u)) onehot[threadIdx.x] = onehot_entry(threadIdx.x);
-    __syncthreads();
     const uint32_t lane = threadIdx.x & 63;
     __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
     Stage st{stage_lds[threadIdx.x >> 6], 0u};
@@ -571,7 +604,7 @@ __global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs 
                 t0 = k * q.chunk_tiles;
                 t1 = min(q.ntiles, t0 + q.chunk_tiles);
             }
-            filter_item<BF, MASK>(a, st, onehot, grp, t0, t1, lane);
+            filter_item<BF, MASK>(a, st, grp, t0, t1, lane);
         }
     }
     stage_flush(a, st);
@@ -899,10 +932,12 @@ struct GatherArgs {
     const uint4* lib_b;
     const float* lib_w;                   // fp32: [block][position][lane]
     const double* lib_band;               // per library slot, any block
+    const double* lib_band_hi;            // per library slot, any block (hi-only stage)
     uint4* out_a;
     uint4* out_b;
     float* out_w;
     double* out_band;
+    double* out_band_hi;
 };
 
 template <bool BF>
@@ -941,6 +976,7 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs g)
     for (uint64_t s = t0; s < (uint64_t)nblk * cpb; s += stride) {
         const uint32_t lib = g.slot_map[s];
         g.out_band[s] = lib != kNoSlot ? g.lib_band[lib] : 0.0;
+        g.out_band_hi[s] = lib != kNoSlot ? g.lib_band_hi[lib] : 0.0;
     }
 }
 
