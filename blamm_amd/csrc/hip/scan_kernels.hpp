@@ -112,11 +112,12 @@ __host__ __device__ constexpr uint32_t group_pack(uint32_t blk0, uint32_t nch, u
 // own LDS slice and reserves global space with one atomic per flush.  Only
 // the owning wave touches its slice; LDS operations of one wave complete in
 // program order, so no barrier is needed.
-constexpr int kStage = 512;
+constexpr int kStage = 256;
 
 struct Stage {
     uint2* buf;        // this wave's LDS slice
     uint32_t n;        // staged entries (wave-uniform)
+    uint4* lo;         // bf16: this wave's lo fragments (LDS), [block][chunk][lane]
 };
 
 __device__ __forceinline__ void stage_flush(const FilterArgs& a, Stage& st)
@@ -181,12 +182,12 @@ __device__ __forceinline__ uint32_t funnel32(uint32_t lo, uint32_t hi, uint32_t 
 // prefetches the next group's words one group ahead.
 constexpr int kGroup = 4;
 
-// Lo products of the bf16 filter: 1 = only for tile blocks whose hi-only
-// score can still reach the threshold (one accumulator set per wave), 0 =
-// always (two accumulator sets, software-pipelined).  1 is about 30% faster
-// at p = 1e-4.
+// Lo products of the bf16 filter: 0 = always (two accumulator sets,
+// software-pipelined); 1 = only for tile blocks whose hi-only score can still
+// reach the threshold (one accumulator set per wave); 2 = as 1, with two
+// accumulator sets (software-pipelined) and the lo fragments kept in LDS.
 #ifndef BLAMM_LAZY_LO
-#define BLAMM_LAZY_LO 1
+#define BLAMM_LAZY_LO 2
 #endif
 
 // ---------------------------------------------------------------------------
@@ -281,8 +282,9 @@ __device__ __forceinline__ float tile_max(const f32x16& v)
 }
 
 // Finish a tile whose hi products are in acc (read no sooner than the MFMAs
-// allow: callers issue other MFMAs or a guard in between).
-template <int NCH, int NB, bool MASK>
+// allow: callers issue other MFMAs or a guard in between).  The lo fragments
+// come from registers (bl) or, when LO_LDS, from the wave's LDS slice.
+template <int NCH, int NB, bool MASK, bool LO_LDS>
 __device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st, f32x16 (&acc)[NB],
                                                  const bf16x8 (&bl)[NB][NCH], const float (&th)[NB],
                                                  const float (&thh)[NB], uint64_t bits, uint32_t mb, uint32_t q0,
@@ -298,7 +300,10 @@ __device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st,
         const bf16x8 A = onehot_chunk<MASK>(bits, mb, j);
 #pragma unroll
         for (int b = 0; b < NB; ++b)
-            if (need & (1u << b)) acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, bl[b][j], acc[b], 0, 0, 0);
+            if (need & (1u << b)) {
+                const bf16x8 B = LO_LDS ? __builtin_bit_cast(bf16x8, st.lo[(b * NCH + j) * 64 + lane]) : bl[b][j];
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc[b], 0, 0, 0);
+            }
     }
     mfma_result_guard();
     const uint32_t qlane = q0 + 4 * (lane >> 5);
@@ -325,7 +330,8 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
             const size_t idx = ((size_t)(blk0 + b) * kMaxChunks + j) * 64 + lane;
             const uint4 h = fh[idx], l = fl[idx];
             bh[b][j] = __builtin_bit_cast(bf16x8, h);
-            bl[b][j] = __builtin_bit_cast(bf16x8, l);
+            if (BLAMM_LAZY_LO == 2) st.lo[(b * NCH + j) * 64 + lane] = l;   // each lane reads back only its own
+            else bl[b][j] = __builtin_bit_cast(bf16x8, l);
         }
         const uint32_t slot = (blk0 + b) * kBf16Cols + (lane & 31);
         th[b] = thr_minus(a.thr[slot], a.band[slot]);
@@ -337,7 +343,48 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
     const uint32_t sh = 2 * (off & 31), sh_inv = 63 - sh;
     const uint64_t* __restrict__ pk = a.packed + woff;
     const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + woff;
-#if BLAMM_LAZY_LO
+#if BLAMM_LAZY_LO == 2
+    // hi products of tile t are issued before tile t-1 is finished
+    static_assert(NB * NCH <= 2 * kMaxChunks, "lo fragments exceed the wave's LDS slice");
+    f32x16 accA[NB], accB[NB];
+    uint64_t bA, bB = 0;
+    uint32_t mA = 0, mB = 0, m0 = 0, m1 = 0, m2 = 0;
+    bA = funnel64(pk[t0], pk[t0 + 1], sh, sh_inv);
+    if (MASK) mA = funnel32(mk[t0], mk[t0 + 1], off & 31);
+    bf16_hi_tile<NCH, NB, MASK>(accA, bh, bA, mA);
+    uint64_t w0 = pk[t0 + 1], w1 = pk[t0 + 2], w2 = pk[t0 + 3];
+    if (MASK) { m0 = mk[t0 + 1]; m1 = mk[t0 + 2]; m2 = mk[t0 + 3]; }
+    uint32_t t = t0 + 1;
+    bool pendingA = true;           // accA holds tile t-1, not yet finished
+    for (; t + 1 < t1; t += 2) {
+        const uint64_t w3 = pk[t + 3], w4 = pk[t + 4];
+        uint32_t m3 = 0, m4 = 0;
+        if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
+        bB = funnel64(w0, w1, sh, sh_inv);
+        if (MASK) mB = funnel32(m0, m1, off & 31);
+        bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
+        bf16_finish_tile<NCH, NB, MASK, true>(a, st, accA, bl, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
+        bA = funnel64(w1, w2, sh, sh_inv);
+        if (MASK) mA = funnel32(m1, m2, off & 31);
+        bf16_hi_tile<NCH, NB, MASK>(accA, bh, bA, mA);
+        bf16_finish_tile<NCH, NB, MASK, true>(a, st, accB, bl, th, thh, bB, mB, t * kBf16Tile, blk0, lane);
+        w0 = w2; w1 = w3; w2 = w4;
+        if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
+    }
+    if (t < t1) {                   // one tile left: t
+        bB = funnel64(w0, w1, sh, sh_inv);
+        if (MASK) mB = funnel32(m0, m1, off & 31);
+        bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
+        bf16_finish_tile<NCH, NB, MASK, true>(a, st, accA, bl, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
+        pendingA = false;
+        mfma_result_guard();
+        bf16_finish_tile<NCH, NB, MASK, true>(a, st, accB, bl, th, thh, bB, mB, t * kBf16Tile, blk0, lane);
+    }
+    if (pendingA) {
+        mfma_result_guard();
+        bf16_finish_tile<NCH, NB, MASK, true>(a, st, accA, bl, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
+    }
+#elif BLAMM_LAZY_LO
     // one accumulator set: hi products of a tile, then its finish (the other
     // resident wave keeps the matrix cores busy meanwhile)
     f32x16 acc[NB];
@@ -350,7 +397,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         const uint32_t mb = MASK ? funnel32(cm0, cm1, off & 31) : 0u;
         bf16_hi_tile<NCH, NB, MASK>(acc, bh, bits, mb);
         mfma_result_guard();
-        bf16_finish_tile<NCH, NB, MASK>(a, st, acc, bl, th, thh, bits, mb, t * kBf16Tile, blk0, lane);
+        bf16_finish_tile<NCH, NB, MASK, false>(a, st, acc, bl, th, thh, bits, mb, t * kBf16Tile, blk0, lane);
         c0 = c1; c1 = c2;
         if (MASK) { cm0 = cm1; cm1 = cm2; }
     }
@@ -574,7 +621,10 @@ __global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs 
 {
     const uint32_t lane = threadIdx.x & 63;
     __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
-    Stage st{stage_lds[threadIdx.x >> 6], 0u};
+    // bf16: per-wave lo fragments (2 blocks x 8 chunks x 64 lanes)
+    __shared__ uint4 lo_lds[(BF && BLAMM_LAZY_LO == 2) ? kWavesPerBlock * 2 * kMaxChunks * 64 : 1];
+    Stage st{stage_lds[threadIdx.x >> 6], 0u,
+             (BF && BLAMM_LAZY_LO == 2) ? lo_lds + (threadIdx.x >> 6) * 2 * kMaxChunks * 64 : lo_lds};
     const uint32_t home = blockIdx.x % kXcds;
     for (uint32_t hop = 0; hop < (uint32_t)kXcds; ++hop) {
         const uint32_t x = (home + hop) % kXcds;
