@@ -295,15 +295,18 @@ __device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st,
     for (int b = 0; b < NB; ++b)
         if (__ballot(tile_max<NB>(acc[b]) >= thh[b])) need |= 1u << b;
     if (need == 0) return;
+    // one uniform branch per block around its whole chunk loop (a branch per
+    // MFMA would copy the accumulator at every join)
 #pragma unroll
-    for (int j = 0; j < NCH; ++j) {
-        const bf16x8 A = onehot_chunk<MASK>(bits, mb, j);
+    for (int b = 0; b < NB; ++b) {
+        if (!(need & (1u << b))) continue;
+        bf16x8 B[NCH];
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
-            if (need & (1u << b)) {
-                const bf16x8 B = LO_LDS ? __builtin_bit_cast(bf16x8, st.lo[(b * NCH + j) * 64 + lane]) : bl[b][j];
-                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc[b], 0, 0, 0);
-            }
+        for (int j = 0; j < NCH; ++j)
+            B[j] = LO_LDS ? __builtin_bit_cast(bf16x8, st.lo[(b * NCH + j) * 64 + lane]) : bl[b][j];
+#pragma unroll
+        for (int j = 0; j < NCH; ++j)
+            acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(onehot_chunk<MASK>(bits, mb, j), B[j], acc[b], 0, 0, 0);
     }
     mfma_result_guard();
     const uint32_t qlane = q0 + 4 * (lane >> 5);
