@@ -82,6 +82,37 @@ def test_ties_on_exact_grid(scanner, orc, prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
+def test_thresholds_at_window_scores_with_large_lo_parts(scanner, orc, prec):
+    """Weights whose bf16 lo parts are large (positive, near half an ulp of
+    hi) and thresholds equal to the largest float not above a chosen window's
+    FP64 score: the window must be reported.  Its hi-only score falls short of
+    the threshold by about the sum of the lo parts, so a lazy-lo bound that
+    left the lo terms out would drop it."""
+    rng = np.random.default_rng(77)
+    n = 40_000
+    codes = rng.integers(0, 4, size=n)
+    text = synth.CODE2BASE[codes].tobytes()
+    pwms, thr = [], []
+    for _ in range(48):
+        L = int(rng.integers(1, 33))
+        hi = rng.uniform(1.0, 2.0, size=(L, 4)).astype(np.float32)
+        # bf16-representable hi, lo just under half a bf16 ulp of hi
+        hi = (hi.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32)
+        ulp = np.float32(2.0 ** -7) * np.float32(2.0) ** np.floor(np.log2(hi))
+        w = (hi + np.float32(0.49) * ulp).astype(np.float32)
+        pwms.append(w)
+        q = int(rng.integers(0, n - 40))
+        s64 = sum(float(w[p, codes[q + p]]) for p in range(L))
+        t = np.float32(s64)
+        if float(t) > s64:
+            t = np.nextafter(t, np.float32(-np.inf))
+        thr.append(t)
+    say(f"lo-heavy ties prec={prec}")
+    n_hits = check_against_oracle(scanner, orc, text, [0], pwms, np.asarray(thr, np.float32), prec)
+    assert n_hits >= 48
+
+
+@pytest.mark.parametrize("prec", PRECS)
 def test_edge_cases(scanner, orc, prec):
     rng = np.random.default_rng(41)
     w1 = [np.array([[1.0, -1.0, 0.5, -0.5]], np.float32)]
