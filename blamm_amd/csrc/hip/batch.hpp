@@ -86,7 +86,8 @@ struct blamm_hip_batch {
     DevBuf<double> band, band_hi;   // per batch slot
     DevBuf<uint32_t> req_start, req_end, req_count;
     DevBuf<unsigned long long> req_off, cursor;
-    DevBuf<uint2> items, cand;
+    DevBuf<uint2> items;
+    DevBuf<uint4> cand;
     DevBuf<uint4> chunks;
     DevBuf<BatchDesc> desc;
     DevBuf<unsigned long long> counters;
@@ -308,7 +309,7 @@ uint64_t blamm_hip_batch_device_bytes(const blamm_hip_batch* b)
     return b->packed.n * 8 + b->lcmask.n * 8 + b->runs.n * 8 + b->thr.n * 4 + b->slot_map.n * 4 + b->blk_req.n * 4 +
            b->groups.n * 4 + (b->frag_a.n + b->frag_b.n) * 16 + (b->band.n + b->band_hi.n) * 8 +
            (b->req_start.n + b->req_end.n + b->req_count.n) * 4 + (b->req_off.n + b->cursor.n) * 8 + b->items.n * 8 +
-           b->cand.n * 8 + b->chunks.n * 16 + (b->hits.n + b->grouped.n) * sizeof(blamm_hit) + sizeof(BatchDesc) +
+           b->cand.n * 16 + b->chunks.n * 16 + (b->hits.n + b->grouped.n) * sizeof(blamm_hit) + sizeof(BatchDesc) +
            kCntSlots * 8;
 }
 

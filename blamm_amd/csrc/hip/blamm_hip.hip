@@ -152,7 +152,7 @@ struct blamm_hip_ctx {
     DevBuf<uint32_t> slot_col, groups_d;
 
     // scratch
-    DevBuf<uint2> cand;
+    DevBuf<uint4> cand;
     DevBuf<unsigned long long> counters;
 
     // host-buffer staging
@@ -438,7 +438,7 @@ int relayout(blamm_hip_ctx* c)
 }
 
 FilterArgs filter_args(const blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, const float* thr,
-                       uint2* cand, uint64_t cand_cap, unsigned long long* counters)
+                       uint4* cand, uint64_t cand_cap, unsigned long long* counters)
 {
     FilterArgs a;
     a.packed = packed;
@@ -455,7 +455,7 @@ FilterArgs filter_args(const blamm_hip_ctx* c, const uint64_t* packed, const uin
     return a;
 }
 
-FinalizeArgs finalize_args(const blamm_hip_ctx* c, const uint2* cand, uint64_t cand_cap, unsigned long long* counters,
+FinalizeArgs finalize_args(const blamm_hip_ctx* c, const uint4* cand, uint64_t cand_cap, unsigned long long* counters,
                            const uint64_t* packed, const uint64_t* lcmask, const uint64_t* runs)
 {
     FinalizeArgs f;

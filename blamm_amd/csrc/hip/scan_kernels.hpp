@@ -74,7 +74,7 @@ struct FilterArgs {
     const double* band;          // per slot: filter band (MFMA score vs FP64 score)
     const double* band_hi;       // per slot: band of the hi-only score (bf16 lazy-lo stage)
     const uint32_t* groups;      // column groups, see group_pack()
-    uint2* cand;                 // (first window of a lane, slot)
+    uint4* cand;                 // (first window of a lane, slot, rows above the threshold, 0)
     unsigned long long* counters;
     uint64_t cand_cap;
 };
@@ -112,10 +112,10 @@ __host__ __device__ constexpr uint32_t group_pack(uint32_t blk0, uint32_t nch, u
 // own LDS slice and reserves global space with one atomic per flush.  Only
 // the owning wave touches its slice; LDS operations of one wave complete in
 // program order, so no barrier is needed.
-constexpr int kStage = 256;
+constexpr int kStage = 128;
 
 struct Stage {
-    uint2* buf;        // this wave's LDS slice
+    uint4* buf;        // this wave's LDS slice
     uint32_t n;        // staged entries (wave-uniform)
     uint4* lo;         // bf16: this wave's lo fragments (LDS), [block][chunk][lane]
 };
@@ -135,19 +135,28 @@ __device__ __forceinline__ void stage_flush(const FilterArgs& a, Stage& st)
 }
 
 // Record the lanes whose accumulator maximum clears the filter threshold.
-// A record (first window of the lane, slot) stands for all windows the lane
-// holds (16 for the bf16 tile, 4 for the fp32 sub-tile); finalize_kernel
-// resolves them exactly.  Keeping row resolution out of the filter keeps its
-// rare path to a handful of instructions.
-__device__ __forceinline__ void emit_lane(const FilterArgs& a, Stage& st, bool pred, uint32_t qlane,
-                                          uint32_t slot)
+// A record (first window of the lane, slot, row mask) stands for the lane's
+// windows (16 rows of the bf16 tile, 4 of the fp32 sub-tile) whose MFMA score
+// clears the filter threshold; finalize_kernel decides them exactly.  The row
+// mask is built only when some lane of the wave passes (rare), so the common
+// path stays a max, a compare and a ballot.
+template <int NR>
+__device__ __forceinline__ void emit_rows(const FilterArgs& a, Stage& st, const float (&acc)[NR], float th,
+                                          uint32_t qlane, uint32_t slot)
 {
+    float mx = acc[0];
+#pragma unroll
+    for (int r = 1; r < NR; ++r) mx = fmaxf(mx, acc[r]);
+    const bool pred = mx >= th;
     const uint64_t m = __ballot(pred);
     if (m) {
+        uint32_t rows = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) rows |= (acc[r] >= th ? 1u : 0u) << r;
         if (st.n > (uint32_t)(kStage - 64)) stage_flush(a, st);
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        if (pred) st.buf[st.n + rank] = make_uint2(qlane, slot);
+        if (pred) st.buf[st.n + rank] = make_uint4(qlane, slot, rows, 0u);
         st.n += (uint32_t)__popcll(m);
     }
 }
@@ -243,10 +252,10 @@ __device__ __forceinline__ void bf16_epilogue(const FilterArgs& a, Stage& st, co
     const uint32_t qlane = q0 + 4 * (lane >> 5);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        float mx = acc[b][0];
+        float v[16];
 #pragma unroll
-        for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[b][r]);
-        emit_lane(a, st, mx >= th[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
+        for (int r = 0; r < 16; ++r) v[r] = acc[b][r];
+        emit_rows<16>(a, st, v, th[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
     }
 }
 
@@ -312,8 +321,12 @@ __device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st,
     const uint32_t qlane = q0 + 4 * (lane >> 5);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-        if (need & (1u << b))
-            emit_lane(a, st, tile_max<NB>(acc[b]) >= th[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
+        if (need & (1u << b)) {
+            float v[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = acc[b][r];
+            emit_rows<16>(a, st, v, th[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
+        }
 }
 
 // Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1.  Lane-private bases
@@ -481,8 +494,8 @@ __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, con
     mfma_result_guard();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const float mx = fmaxf(fmaxf(acc[b][0], acc[b][1]), fmaxf(acc[b][2], acc[b][3]));
-        emit_lane(a, st, mx >= th[b], qlane, (blk0 + b) * kFp32Cols + (lane & 15));
+        const float v[4] = {acc[b][0], acc[b][1], acc[b][2], acc[b][3]};
+        emit_rows<4>(a, st, v, th[b], qlane, (blk0 + b) * kFp32Cols + (lane & 15));
     }
 }
 
@@ -623,7 +636,7 @@ template <bool BF, bool MASK, bool LIST>
 __global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs a, WorkQueue q, ListArgs l)
 {
     const uint32_t lane = threadIdx.x & 63;
-    __shared__ uint2 stage_lds[kWavesPerBlock][kStage];
+    __shared__ uint4 stage_lds[kWavesPerBlock][kStage];
     // bf16: per-wave lo fragments (2 blocks x 8 chunks x 64 lanes)
     __shared__ uint4 lo_lds[(BF && BLAMM_LAZY_LO == 2) ? kWavesPerBlock * 2 * kMaxChunks * 64 : 1];
     Stage st{stage_lds[threadIdx.x >> 6], 0u,
@@ -664,7 +677,7 @@ __global__ __launch_bounds__(kThreads) void filter_persistent_kernel(FilterArgs 
 }
 
 struct FinalizeArgs {
-    const uint2* cand;
+    const uint4* cand;
     const unsigned long long* counters;   // kCntCand read, kCntHits/kCntBand written
     unsigned long long* counters_rw;
     uint64_t cand_cap;
@@ -742,11 +755,12 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
         bool have = idx < ncand;
         // slot: threshold slot of the record; lib: library slot holding the
         // column's weights (the same slot outside the list form)
-        uint32_t q0 = 0, slot = 0, lib = 0, req = 0, qend = nbases;
+        uint32_t q0 = 0, slot = 0, lib = 0, req = 0, qend = nbases, rows = 0;
         if (have) {
-            const uint2 cd = a.cand[idx];
+            const uint4 cd = a.cand[idx];
             q0 = cd.x;
             slot = lib = cd.y;
+            rows = cd.z;
             if constexpr (LIST) {
                 constexpr uint32_t cpb = NR == 16 ? kBf16Cols : kFp32Cols;
                 req = a.blk_req[slot / cpb];
@@ -755,15 +769,12 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
                 if (lib == kNoSlot) have = false, lib = 0;   // empty slots never pass the filter
             }
         }
+        if (!have) rows = 0;
         const int L = have ? a.slot_len[lib] : 0;
         const float* w = a.slot_w + (size_t)lib * kMaxLen * 4;
-        // packed bases q0 .. q0+63 and their lowercase bits
-        uint64_t lo = 0, hi = 0, mlo = 0;
         uint32_t nb1 = 0;
         if (have) {
-            lo = load_bits64(a.packed, 2ull * q0);
-            hi = load_bits64(a.packed, 2ull * q0 + 64);
-            if (a.lcmask) mlo = load_bits64(a.lcmask, q0);
+            // first run start after q0
             uint32_t l = 0, h = nruns;
             while (l < h) {
                 const uint32_t mid = (l + h) >> 1;
@@ -771,34 +782,35 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             }
             nb1 = l < nruns ? (uint32_t)a.run_starts[l] : nbases;
         }
-        float s[NR];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) s[r] = 0.f;
-        uint64_t wl = lo, wh = hi, ml = mlo;
-        for (int p = 0; p < L; ++p) {
-            const float4 wp = *(const float4*)(w + 4 * p);
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const uint32_t o = record_offset<NR>(r);
-                const uint32_t b = (uint32_t)(wl >> (2 * o)) & 3u;
-                const float x = b == 0 ? wp.x : b == 1 ? wp.y : b == 2 ? wp.z : wp.w;
-                if (!((ml >> o) & 1)) s[r] += x;
-            }
-            wl = (wl >> 2) | (wh << 62);
-            wh >>= 2;
-            ml >>= 1;
-        }
         const float thr = have ? a.slot_thr[slot] : 0.f;
         const double seq = have ? a.slot_seq[lib] : 0.0;
         const float shi = thr_plus(thr, seq);
         const float slo = thr_minus(thr, seq);
         const uint32_t col = have ? a.slot_col[lib] : 0u;
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
+        // every lane takes its next marked window until no lane has one left
+        // (uniform loop: the hit staging below stays wave-wide)
+        while (__ballot(rows != 0u)) {
+            const bool act = rows != 0u;
+            const uint32_t r = act ? (uint32_t)__builtin_ctz(rows) : 0u;
+            rows &= rows - 1u;
             const uint32_t q = q0 + record_offset<NR>(r);
+            uint64_t bits = 0, mbits = 0;
+            if (act) {
+                bits = load_bits64(a.packed, 2ull * q);
+                if (a.lcmask) mbits = load_bits64(a.lcmask, q);
+            }
+            // left-to-right FP32 sum of the window
+            float s = 0.f;
+            const int Lw = act ? L : 0;
+            for (int p = 0; p < Lw; ++p) {
+                const float4 wp = *(const float4*)(w + 4 * p);
+                const uint32_t b = (uint32_t)(bits >> (2 * p)) & 3u;
+                const float x = b == 0 ? wp.x : b == 1 ? wp.y : b == 2 ? wp.z : wp.w;
+                if (!((mbits >> p) & 1)) s += x;
+            }
             bool hit = false;
-            float score = s[r];
-            if (have && q < qend && s[r] >= slo) {
+            float score = s;
+            if (act && q < qend && s >= slo) {
                 // run-boundary rule: next run start after q
                 uint32_t nb = nb1;
                 if (q >= nb1) {
@@ -810,15 +822,13 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
                     nb = l < nruns ? (uint32_t)a.run_starts[l] : nbases;
                 }
                 if ((uint64_t)q + (uint64_t)L <= (uint64_t)min(nb, qend)) {
-                    if (s[r] >= shi) {
+                    if (s >= shi) {
                         hit = true;
                     } else {
                         // within the FP32 band: FP64 left-to-right sum
-                        const uint64_t bits = load_bits64(a.packed, 2ull * q);
-                        const uint64_t mb = a.lcmask ? load_bits64(a.lcmask, q) : 0ull;
                         double d = 0.0;
                         for (int p = 0; p < L; ++p) {
-                            if ((mb >> p) & 1) continue;
+                            if ((mbits >> p) & 1) continue;
                             d += (double)w[4 * p + ((bits >> (2 * p)) & 3)];
                         }
                         atomicAdd(&a.counters_rw[kCntBand], 1ull);
