@@ -372,7 +372,8 @@ def serve_leg(args):
         f"{sum(r.nbases for r, _ in pool) / len(pool) / 1e6:.2f} Mbp per request ({time.time() - t0:.1f}s)")
     sc = ba.HipScanner(0)
     sc.set_library(P, L, thr[1e-4], ba.PREC_BF16X2)
-    srv = ba.Server(sc, hbm_budget_bytes=args.serve_budget_gib << 30, slots=2, max_batch_requests=64)
+    srv = ba.Server(sc, hbm_budget_bytes=args.serve_budget_gib << 30, slots=2, max_batch_requests=64,
+                    max_batch_bases=args.serve_batch_mbp * 1_000_000)
 
     def run(arrivals):
         """Submit pool requests at the given offsets (s); returns latencies (ms),
@@ -493,6 +494,7 @@ def main():
     ap.add_argument("--serve-max-seqs", type=int, default=5000)
     ap.add_argument("--serve-budget-gib", type=int, default=32)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-batch-mbp", type=int, default=32, help="bases per serving batch (Mbp)")
     ap.add_argument("--serve-loads", type=lambda v: [float(x) for x in v.split(",")], default=[0.3, 0.6, 0.85])
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for scaling FETCH_SIZE/WRITE_SIZE)")

@@ -69,7 +69,7 @@ class Request(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("slots", ctypes.c_uint32),
-                ("max_batch_requests", ctypes.c_uint32)]
+                ("max_batch_requests", ctypes.c_uint32), ("max_batch_bases", ctypes.c_uint64)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -439,11 +439,12 @@ class Batch:
 class Server:
     """Continuous-batching server over the scanner's resident library."""
 
-    def __init__(self, scanner: HipScanner, hbm_budget_bytes: int = 0, slots: int = 0, max_batch_requests: int = 0):
+    def __init__(self, scanner: HipScanner, hbm_budget_bytes: int = 0, slots: int = 0, max_batch_requests: int = 0,
+                 max_batch_bases: int = 0):
         self.scanner = scanner
         self.lib = scanner.lib
         self.s = ctypes.c_void_p()
-        cfg = ServerConfig(hbm_budget_bytes, slots, max_batch_requests)
+        cfg = ServerConfig(hbm_budget_bytes, slots, max_batch_requests, max_batch_bases)
         _check(self.lib.blamm_hip_server_create(scanner.ctx, ctypes.byref(cfg), ctypes.byref(self.s)))
         self._keep = {}
 

@@ -143,7 +143,7 @@ void serve_loop(blamm_hip_server* S)
             while (!S->queue.empty() && ids.size() < S->cfg.max_batch_requests) {
                 const ServeRequest& q = S->reqs[S->queue.front()];
                 if (!ids.empty() && (bytes + q.bytes > S->slot_budget ||
-                                     bases + q.req.nbases + kReqAlign > kBatchMaxBases))
+                                     bases + q.req.nbases + kReqAlign > S->cfg.max_batch_bases))
                     break;
                 bytes += q.bytes;
                 bases += q.req.nbases + kReqAlign;
@@ -206,6 +206,8 @@ int blamm_hip_server_create(blamm_hip_ctx* c, const blamm_server_config* cfg, bl
     if (k.slots == 0) k.slots = 2;
     if (k.slots > 4) return fail(BLAMM_ERR_ARG, "server_create: at most 4 slots");
     if (k.max_batch_requests == 0) k.max_batch_requests = 256;
+    if (k.max_batch_bases == 0) k.max_batch_bases = 1ull << 26;
+    k.max_batch_bases = std::min<uint64_t>(k.max_batch_bases, kBatchMaxBases);
     if (k.max_batch_requests >= (1u << 16)) return fail(BLAMM_ERR_ARG, "server_create: max_batch_requests too large");
     blamm_hip_server* S = new blamm_hip_server();
     S->ctx = c;

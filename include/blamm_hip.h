@@ -240,6 +240,9 @@ typedef struct blamm_server_config {
                                      split evenly over the slots */
     uint32_t slots;               /* batches in flight, 1..4 (0 = 2) */
     uint32_t max_batch_requests;  /* requests per batch (0 = 256) */
+    uint64_t max_batch_bases;     /* bases per batch (0 = 2^26): smaller batches let
+                                     one batch's hit read-back overlap the next
+                                     batch's scan */
 } blamm_server_config;
 
 typedef struct blamm_request_info {
