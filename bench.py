@@ -494,7 +494,7 @@ def main():
     ap.add_argument("--serve-max-seqs", type=int, default=5000)
     ap.add_argument("--serve-budget-gib", type=int, default=32)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
-    ap.add_argument("--serve-batch-mbp", type=int, default=32, help="bases per serving batch (Mbp)")
+    ap.add_argument("--serve-batch-mbp", type=int, default=16, help="bases per serving batch (Mbp)")
     ap.add_argument("--serve-loads", type=lambda v: [float(x) for x in v.split(",")], default=[0.3, 0.6, 0.85])
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for scaling FETCH_SIZE/WRITE_SIZE)")

@@ -206,7 +206,7 @@ int blamm_hip_server_create(blamm_hip_ctx* c, const blamm_server_config* cfg, bl
     if (k.slots == 0) k.slots = 2;
     if (k.slots > 4) return fail(BLAMM_ERR_ARG, "server_create: at most 4 slots");
     if (k.max_batch_requests == 0) k.max_batch_requests = 256;
-    if (k.max_batch_bases == 0) k.max_batch_bases = 1ull << 26;
+    if (k.max_batch_bases == 0) k.max_batch_bases = 1ull << 24;
     k.max_batch_bases = std::min<uint64_t>(k.max_batch_bases, kBatchMaxBases);
     if (k.max_batch_requests >= (1u << 16)) return fail(BLAMM_ERR_ARG, "server_create: max_batch_requests too large");
     blamm_hip_server* S = new blamm_hip_server();

@@ -127,4 +127,30 @@ int blamm_hip_autotune(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t pack
     return BLAMM_OK;
 }
 
+int blamm_hip_autotune_host(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
+                            const uint64_t* run_starts, uint64_t nruns, const char* cache_path,
+                            blamm_tuning* chosen, double* chosen_ms)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "autotune: null context");
+    if (nbases == 0 || !packed) return fail(BLAMM_ERR_ARG, "autotune: empty sample");
+    if (nruns > 0 && !run_starts) return fail(BLAMM_ERR_ARG, "autotune: null run starts");
+    if (nbases >= (1ull << 32) - 512) return fail(BLAMM_ERR_ARG, "autotune: sample must be < 2^32 - 512 bases");
+    HIP_TRY(hipSetDevice(c->device));
+    const uint64_t pw = blamm_hip_packed_words(nbases), mw = blamm_hip_mask_words(nbases);
+    HIP_TRY(c->packed.reserve(pw));
+    HIP_TRY(hipMemsetAsync(c->packed.p, 0, pw * 8, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->packed.p, packed, ((nbases + 31) / 32) * 8, hipMemcpyHostToDevice, c->stream));
+    if (lcmask) {
+        HIP_TRY(c->lcmask.reserve(mw));
+        HIP_TRY(hipMemsetAsync(c->lcmask.p, 0, mw * 8, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->lcmask.p, lcmask, ((nbases + 63) / 64) * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_TRY(c->runs.reserve(nruns + 1));
+    if (nruns > 0)
+        HIP_TRY(hipMemcpyAsync(c->runs.p, run_starts, nruns * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return blamm_hip_autotune(c, c->packed.p, pw, lcmask ? c->lcmask.p : nullptr, mw, nbases, c->runs.p, nruns,
+                              cache_path, chosen, chosen_ms);
+}
+
 }  // extern "C"

@@ -106,7 +106,9 @@ void scan_usage()
                  "  -d\t--device\tHIP device index [default = 0]\n\n"
                  " [file_options]\n"
                  "  -H\t--histdir\tdirectory where the histogram file(s) are stored [default = .]\n"
-                 "  -o\t--output\tfilename for the motif occurrences [default = occurences.txt]\n\n";
+                 "  -o\t--output\tfilename for the motif occurrences [default = occurences.txt]\n"
+                 "  -T\t--tune\t\tpick the GPU tile shapes by timing a sample of each group;\n"
+                 "    \t\t\tresults are cached in the given file and reused\n\n";
 }
 
 std::string dir_with_slash(std::string d)
@@ -279,7 +281,7 @@ int scan_main(int argc, char** argv)
     bool revCompl = false, absSpec = false, relSpec = false, pSpec = false;
     float absTh = 0.0f, relTh = 0.95f, pvalue = 0.0001f;
     size_t numThreads = std::max(1u, std::thread::hardware_concurrency());
-    std::string histdir, outName = "occurrences.txt";
+    std::string histdir, outName = "occurrences.txt", tuneCache;
     int precision = BLAMM_PREC_BF16X2, device = 0;
     for (int i = 2; i < argc - 2; ++i) {
         const std::string a(argv[i]);
@@ -319,6 +321,8 @@ int scan_main(int argc, char** argv)
             else throw std::runtime_error("precision must be fp32 or bf16x2");
         } else if ((a == "-d" || a == "--device") && more) {
             device = std::atoi(argv[++i]);
+        } else if ((a == "-T" || a == "--tune") && more) {
+            tuneCache = argv[++i];
         } else {
             scan_usage();
             return EXIT_FAILURE;
@@ -419,6 +423,26 @@ int scan_main(int argc, char** argv)
         // lmax - 1 bases; hits starting in a halo belong to the next segment.
         const uint64_t seg = 1ull << 30;
         const uint64_t halo = lmax - 1;
+        if (!tuneCache.empty() && s.n > 0) {
+            // tile shapes from a sample of this group (the first 16 Mbp)
+            const uint64_t m = std::min<uint64_t>(s.n, 1ull << 24);
+            std::vector<uint64_t> runs;
+            for (size_t r = 0; r < s.runStart.size(); ++r)
+                if (s.runStart[r] > 0 && s.runStart[r] < m) runs.push_back(s.runStart[r]);
+            std::vector<uint64_t> codes(blamm_hip_packed_words(m), 0), lower;
+            std::memcpy(codes.data(), s.codes.data(), ((m + 31) / 32) * 8);
+            if (s.hasLower()) {
+                lower.assign(blamm_hip_mask_words(m), 0);
+                std::memcpy(lower.data(), s.lower.data(), ((m + 63) / 64) * 8);
+            }
+            blamm_tuning t{};
+            double ms = 0.0;
+            hip_check(blamm_hip_autotune_host(hc.ctx, codes.data(), s.hasLower() ? lower.data() : nullptr, m,
+                                              runs.data(), runs.size(), tuneCache.c_str(), &t, &ms),
+                      "tile-shape tuning");
+            std::cout << "GPU tile shapes: " << t.chunk_tiles << " tiles per item, " << t.blocks_per_wave
+                      << " column blocks per wave, K padded to " << t.kpad_positions << " positions" << std::endl;
+        }
         std::vector<blamm_hit> hits(1u << 20);
         for (uint64_t s0 = 0; s0 < s.n || (s0 == 0 && s.n == 0); s0 += seg) {
             if (s.n == 0) break;

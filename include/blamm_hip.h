@@ -181,6 +181,11 @@ int blamm_hip_autotune(blamm_hip_ctx *ctx, const uint64_t *d_packed, uint64_t pa
                        const uint64_t *d_lcmask, uint64_t mask_words, uint64_t nbases,
                        const uint64_t *d_run_starts, uint64_t nruns, const char *cache_path,
                        blamm_tuning *chosen, double *chosen_ms);
+/* The same on a sample held in host memory (staged through the context's own
+ * device buffers; buffer sizes as for blamm_hip_scan). */
+int blamm_hip_autotune_host(blamm_hip_ctx *ctx, const uint64_t *packed, const uint64_t *lcmask,
+                            uint64_t nbases, const uint64_t *run_starts, uint64_t nruns,
+                            const char *cache_path, blamm_tuning *chosen, double *chosen_ms);
 
 /* ------------------------------------------------------------------------
  * Batches: many requests in one persistent launch.  The reference scans one
@@ -240,7 +245,7 @@ typedef struct blamm_server_config {
                                      split evenly over the slots */
     uint32_t slots;               /* batches in flight, 1..4 (0 = 2) */
     uint32_t max_batch_requests;  /* requests per batch (0 = 256) */
-    uint64_t max_batch_bases;     /* bases per batch (0 = 2^26): smaller batches let
+    uint64_t max_batch_bases;     /* bases per batch (0 = 2^24): smaller batches let
                                      one batch's hit read-back overlap the next
                                      batch's scan */
 } blamm_server_config;

@@ -321,6 +321,31 @@ def test_cli_variants_match_oracle(tmp_path, opts):
     assert open(a / "PWMthresholds.txt").read() == open(b / "PWMthresholds.txt").read()
 
 
+def test_cli_tuning_cache(tmp_path):
+    """`blamm scan -T cache` tunes the tile shapes on a sample of the group,
+    records them, reuses them on the next run, and the output still matches
+    the oracle."""
+    from test_host import _write_synthetic
+    a, b = tmp_path / "prod", tmp_path / "orc"
+    for d in (a, b):
+        d.mkdir()
+        _write_synthetic(str(d), seed=11, lower=False, crlf=False)
+    for d, run in ((str(a), run_blamm), (str(b), run_oracle)):
+        run(["dict", "sequences.mf"], d)
+        run(["hist", "motifs.jaspar", "sequences.mf"], d)
+    run_oracle(["scan", "-rc", "-pt", "0.001", "motifs.jaspar", "sequences.mf"], str(b))
+    counts = []
+    for k in range(2):
+        say(f"cli tune run {k}")
+        r = run_blamm(["scan", "-rc", "-pt", "0.001", "-T", "tune.txt", "motifs.jaspar", "sequences.mf"], str(a))
+        assert "GPU tile shapes" in r.stdout
+        assert_same_occurrences(str(a / "occurrences.txt"), str(b / "occurrences.txt"))
+        counts.append(len(open(a / "tune.txt").read().splitlines()))
+    # one entry per distinct key (device, library, sample size); the second
+    # run finds them all in the cache
+    assert 1 <= counts[0] <= 2 and counts[1] == counts[0], counts
+
+
 def test_full_size_plain_gpu_config(scanner, orc):
     """The 100 Mbp x 2,000-column configuration: both precisions give the same
     hit set; three slices equal the oracle's; sampled hits re-score within the
