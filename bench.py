@@ -166,8 +166,52 @@ def gpu_leg(args, stream, runs, P, lengths, thr, codes, precision, tune_sample):
            "tuning": tuning or ds.sc.tuning().as_dict()}
     # keep a copy of the hits for the cross-precision identity check
     hits = ds.hits(nh)
+    if precision == ds.ba.PREC_BF16X2:
+        out["host_api"] = host_api_leg(ds, codes, n, runs, nh)
     ds.close()
     return out, hits
+
+
+def host_api_leg(ds, codes, n, runs, nh, reps=3):
+    """The same scan through the host-buffer entry point the CLI uses: the
+    packed stream is uploaded and every hit read back inside the timed call."""
+    import ctypes
+    ba = ds.ba
+    lib = ds.sc.lib
+    pw = int(lib.blamm_hip_packed_words(n))
+    cbuf = np.zeros(pw, np.uint64)
+    cbuf[:min(codes.size, pw)] = codes[:pw]
+    rr = np.ascontiguousarray(runs, np.uint64)
+    cap = nh + 4096
+    hits = np.empty(cap, dtype=ba.HIT_DTYPE)
+    cnt = ctypes.c_uint64()
+
+    def call():
+        rc = lib.blamm_hip_scan(ds.sc.ctx, cbuf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), None, n,
+                                rr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), int(rr.size),
+                                hits.ctypes.data, cap, ctypes.byref(cnt))
+        if rc != 0:
+            raise RuntimeError(lib.blamm_hip_last_error().decode())
+
+    hp = ctypes.c_void_p()
+
+    def call_pinned():
+        rc = lib.blamm_hip_scan_pinned(ds.sc.ctx, cbuf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), None, n,
+                                       rr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), int(rr.size),
+                                       ctypes.byref(hp), ctypes.byref(cnt))
+        if rc != 0:
+            raise RuntimeError(lib.blamm_hip_last_error().decode())
+
+    out = {"hits": 0, "upload_bytes": int(pw * 8)}
+    for name, fn in (("ms_per_scan", call), ("ms_per_scan_pinned", call_pinned)):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        out[name] = (time.perf_counter() - t0) / reps * 1e3
+    out["hits"] = int(cnt.value)
+    out["readback_bytes"] = int(cnt.value) * 16
+    return out
 
 
 def hit_keys(h, shift=16):
@@ -559,6 +603,9 @@ def main():
                 "peak_GBs": PEAK_HBM_GBS},
         "scan": bf,
     }
+    if "host_api" in bf:
+        bf["host_api"]["value"] = nbp * ncols / (bf["host_api"]["ms_per_scan"] / 1e3) / 1e9
+        bf["host_api"]["value_pinned"] = nbp * ncols / (bf["host_api"]["ms_per_scan_pinned"] / 1e3) / 1e9
     if fp is not None:
         fval = nbp * ncols / (fp["ms_per_step"] / 1e3) / 1e9
         fach = flops / (fp["filter_ms"] / 1e3) / 1e12

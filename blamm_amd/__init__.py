@@ -131,6 +131,8 @@ def hip_lib() -> ctypes.CDLL:
         lib.blamm_hip_scan_device.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint64, ctypes.c_uint64,
                                               vp, ctypes.c_uint64, vp, ctypes.c_uint64, vp, vp]
         lib.blamm_hip_scan_finish.argtypes = [vp, _u64p]
+        lib.blamm_hip_scan_pinned.argtypes = [vp, _u64p, _u64p, ctypes.c_uint64, _u64p, ctypes.c_uint64,
+                                              ctypes.POINTER(vp), _u64p]
         lib.blamm_hip_last_stats.argtypes = [vp, ctypes.POINTER(ScanStats)]
         lib.blamm_hip_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
         lib.blamm_hip_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
@@ -310,6 +312,27 @@ class HipScanner:
                 continue
             _check(rc)
             return hits[:n.value].copy()
+
+    def scan_pinned(self, codes: np.ndarray, nbases: int, run_starts=None,
+                    lower: Optional[np.ndarray] = None) -> np.ndarray:
+        """Scan a host stream; the engine keeps the hits in its pinned buffer
+        (returned here as a copy)."""
+        pw = int(self.lib.blamm_hip_packed_words(nbases))
+        cbuf = np.zeros(pw, dtype=np.uint64)
+        cbuf[:min(len(codes), pw)] = codes[:pw]
+        mbuf = None
+        if lower is not None:
+            mw = int(self.lib.blamm_hip_mask_words(nbases))
+            mbuf = np.zeros(mw, dtype=np.uint64)
+            mbuf[:min(len(lower), mw)] = lower[:mw]
+        runs = np.ascontiguousarray(run_starts if run_starts is not None else [], dtype=np.uint64)
+        p = ctypes.c_void_p()
+        n = ctypes.c_uint64()
+        _check(self.lib.blamm_hip_scan_pinned(self.ctx, _ptr(cbuf, ctypes.c_uint64),
+                                              _ptr(mbuf, ctypes.c_uint64) if mbuf is not None else None,
+                                              nbases, _ptr(runs, ctypes.c_uint64) if len(runs) else None,
+                                              len(runs), ctypes.byref(p), ctypes.byref(n)))
+        return _hits_array(p.value or 0, int(n.value))
 
     def stats(self) -> ScanStats:
         st = ScanStats()

@@ -158,6 +158,7 @@ struct blamm_hip_ctx {
     // host-buffer staging
     DevBuf<uint64_t> packed, lcmask, runs, nhits_dev;
     DevBuf<blamm_hit> hits;
+    HostBuf<blamm_hit> hits_pinned;   // blamm_hip_scan_pinned's result
 
     // last scan
     bool pending = false;
@@ -553,6 +554,7 @@ int blamm_hip_destroy(blamm_hip_ctx* c)
     c->counters.release();
     c->packed.release(); c->lcmask.release(); c->runs.release(); c->nhits_dev.release();
     c->hits.release();
+    c->hits_pinned.release();
     for (int i = 0; i < 4; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -735,17 +737,15 @@ int blamm_hip_last_stats(blamm_hip_ctx* c, blamm_scan_stats* st)
     return BLAMM_OK;
 }
 
-int blamm_hip_scan(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
-                   const uint64_t* run_starts, uint64_t nruns, blamm_hit* hits, uint64_t cap,
-                   uint64_t* nhits)
+}  // extern "C"
+
+namespace {
+
+// Upload a host stream into the context's staging buffers and run the scan
+// into c->hits (grown until every hit fits); *nhits = hits on the device.
+int scan_staged(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
+                const uint64_t* run_starts, uint64_t nruns, uint64_t cap_limit, uint64_t* nhits)
 {
-    if (!c) return fail(BLAMM_ERR_ARG, "scan: null context");
-    if (!nhits) return fail(BLAMM_ERR_ARG, "scan: null nhits");
-    if (!c->have_lib) return fail(BLAMM_ERR_STATE, "scan: no library uploaded");
-    if (nbases > 0 && !packed) return fail(BLAMM_ERR_ARG, "scan: null packed stream");
-    if (cap > 0 && !hits) return fail(BLAMM_ERR_ARG, "scan: null hit buffer");
-    if (nbases >= (1ull << 32) - 512) return fail(BLAMM_ERR_ARG, "scan: nbases must be < 2^32 - 512 per call");
-    HIP_TRY(hipSetDevice(c->device));
     const uint64_t pw = blamm_hip_packed_words(nbases), mw = blamm_hip_mask_words(nbases);
     HIP_TRY(c->packed.reserve(pw));
     HIP_TRY(hipMemsetAsync(c->packed.p, 0, pw * 8, c->stream));
@@ -759,7 +759,7 @@ int blamm_hip_scan(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcm
     HIP_TRY(c->runs.reserve(nruns + 1));
     if (nruns > 0)
         HIP_TRY(hipMemcpyAsync(c->runs.p, run_starts, nruns * 8, hipMemcpyHostToDevice, c->stream));
-    uint64_t dcap = std::min<uint64_t>(cap, std::max<uint64_t>(c->hits.n, 1u << 16));
+    uint64_t dcap = std::min<uint64_t>(cap_limit, std::max<uint64_t>(c->hits.n, 1u << 16));
     for (int attempt = 0; attempt < 8; ++attempt) {
         HIP_TRY(c->hits.reserve(std::max<uint64_t>(dcap, 1)));
         int rc = blamm_hip_scan_device(c, c->packed.p, pw, lcmask ? c->lcmask.p : nullptr, mw, nbases,
@@ -768,13 +768,12 @@ int blamm_hip_scan(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcm
         uint64_t n = 0;
         rc = blamm_hip_scan_finish(c, &n);
         if (rc == BLAMM_OK) {
-            if (n > 0) HIP_TRY(hipMemcpy(hits, c->hits.p, n * sizeof(blamm_hit), hipMemcpyDeviceToHost));
             *nhits = n;
             return BLAMM_OK;
         }
         if (rc != BLAMM_ERR_OVERFLOW) return rc;
         if (n > dcap) {   // hit buffer too small
-            if (n > cap) {
+            if (n > cap_limit) {
                 *nhits = n;
                 return fail(BLAMM_ERR_OVERFLOW, "scan: hit capacity too small");
             }
@@ -782,6 +781,51 @@ int blamm_hip_scan(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcm
         }
     }
     return fail(BLAMM_ERR_DEVICE, "scan: capacity retries exhausted");
+}
+
+}  // namespace
+
+extern "C" {
+
+int blamm_hip_scan_pinned(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
+                          const uint64_t* run_starts, uint64_t nruns, const blamm_hit** hits, uint64_t* nhits)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "scan: null context");
+    if (!nhits || !hits) return fail(BLAMM_ERR_ARG, "scan: null output");
+    if (!c->have_lib) return fail(BLAMM_ERR_STATE, "scan: no library uploaded");
+    if (nbases > 0 && !packed) return fail(BLAMM_ERR_ARG, "scan: null packed stream");
+    if (nbases >= (1ull << 32) - 512) return fail(BLAMM_ERR_ARG, "scan: nbases must be < 2^32 - 512 per call");
+    HIP_TRY(hipSetDevice(c->device));
+    uint64_t n = 0;
+    BLAMM_TRY(scan_staged(c, packed, lcmask, nbases, run_starts, nruns, UINT64_MAX, &n));
+    if (n > c->hits_pinned.n) HIP_TRY(c->hits_pinned.reserve(n + n / 8));
+    if (n > 0) HIP_TRY(hipMemcpy(c->hits_pinned.p, c->hits.p, n * sizeof(blamm_hit), hipMemcpyDeviceToHost));
+    *hits = c->hits_pinned.p;
+    *nhits = n;
+    return BLAMM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int blamm_hip_scan(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
+                   const uint64_t* run_starts, uint64_t nruns, blamm_hit* hits, uint64_t cap,
+                   uint64_t* nhits)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "scan: null context");
+    if (!nhits) return fail(BLAMM_ERR_ARG, "scan: null nhits");
+    if (!c->have_lib) return fail(BLAMM_ERR_STATE, "scan: no library uploaded");
+    if (nbases > 0 && !packed) return fail(BLAMM_ERR_ARG, "scan: null packed stream");
+    if (cap > 0 && !hits) return fail(BLAMM_ERR_ARG, "scan: null hit buffer");
+    if (nbases >= (1ull << 32) - 512) return fail(BLAMM_ERR_ARG, "scan: nbases must be < 2^32 - 512 per call");
+    HIP_TRY(hipSetDevice(c->device));
+    uint64_t n = 0;
+    const int rc = scan_staged(c, packed, lcmask, nbases, run_starts, nruns, cap, &n);
+    *nhits = n;
+    if (rc != BLAMM_OK) return rc;
+    if (n > 0) HIP_TRY(hipMemcpy(hits, c->hits.p, n * sizeof(blamm_hit), hipMemcpyDeviceToHost));
+    return BLAMM_OK;
 }
 
 }  // extern "C"

@@ -443,7 +443,6 @@ int scan_main(int argc, char** argv)
             std::cout << "GPU tile shapes: " << t.chunk_tiles << " tiles per item, " << t.blocks_per_wave
                       << " column blocks per wave, K padded to " << t.kpad_positions << " positions" << std::endl;
         }
-        std::vector<blamm_hit> hits(1u << 20);
         for (uint64_t s0 = 0; s0 < s.n || (s0 == 0 && s.n == 0); s0 += seg) {
             if (s.n == 0) break;
             const uint64_t payload = std::min(seg, s.n - s0);
@@ -457,18 +456,12 @@ int scan_main(int argc, char** argv)
                 lower.assign(blamm_hip_mask_words(len), 0);
                 std::memcpy(lower.data(), s.lower.data() + s0 / 64, ((len + 63) / 64) * 8);
             }
+            // hits stay in the engine's pinned buffer until the next scan
+            const blamm_hit* hits = nullptr;
             uint64_t nh = 0;
-            int rc;
-            for (;;) {
-                rc = blamm_hip_scan(hc.ctx, codes.data(), s.hasLower() ? lower.data() : nullptr, len, runs.data(),
-                                    runs.size(), hits.data(), hits.size(), &nh);
-                if (rc == BLAMM_ERR_OVERFLOW && nh > hits.size()) {
-                    hits.resize(nh + nh / 8);
-                    continue;
-                }
-                break;
-            }
-            hip_check(rc, "scan");
+            hip_check(blamm_hip_scan_pinned(hc.ctx, codes.data(), s.hasLower() ? lower.data() : nullptr, len,
+                                            runs.data(), runs.size(), &hits, &nh),
+                      "scan");
             std::vector<blamm_hit> kept;
             kept.reserve(nh);
             for (uint64_t i = 0; i < nh; ++i)

@@ -132,6 +132,13 @@ int blamm_hip_scan(blamm_hip_ctx *ctx, const uint64_t *packed, const uint64_t *l
                    uint64_t nbases, const uint64_t *run_starts, uint64_t nruns,
                    blamm_hit *hits, uint64_t cap, uint64_t *nhits);
 
+/* As blamm_hip_scan, but the hits stay in a pinned buffer the context owns
+ * (read back at full PCIe rate, no copy into caller memory): *hits points to
+ * *nhits records, valid until the next call on this context. */
+int blamm_hip_scan_pinned(blamm_hip_ctx *ctx, const uint64_t *packed, const uint64_t *lcmask,
+                          uint64_t nbases, const uint64_t *run_starts, uint64_t nruns,
+                          const blamm_hit **hits, uint64_t *nhits);
+
 /* Enqueue a scan of device-resident buffers on `stream` (a hipStream_t, or
  * NULL for the context's own stream) without synchronising.  Buffer sizes are
  * passed so they can be checked.  The hit count is written to d_nhits

@@ -172,6 +172,20 @@ def test_repeated_scans_are_identical(scanner, orc, prec):
     assert not bad, bad
 
 
+def test_pinned_scan_matches_host_scan(scanner, orc):
+    text, runs, bg = stream_case(131, 90_000, lower_every=11)
+    ms = synth.motifs(30, 4, 28, 0.5, seed=131)
+    pwms, thr = library(ms, bg, pvalue=1e-3)
+    P, lengths, _ = ba.build_pattern_matrix(pwms)
+    scanner.set_library(P, lengths, thr, ba.PREC_BF16X2)
+    packed, lower = ba.pack(text)
+    a = scanner.scan(packed, len(text), runs, lower)
+    for _ in range(2):
+        b = scanner.scan_pinned(packed, len(text), runs, lower)
+        assert np.array_equal(np.sort(a, order=["pos", "col"]), np.sort(b, order=["pos", "col"]))
+    assert hit_set(a) == hit_set(orc.scan_fp64(text, runs, pwms, thr))
+
+
 def test_hit_capacity_overflow_is_reported(scanner):
     text, runs, bg = stream_case(61, 30_000)
     ms = synth.motifs(8, 5, 12, 0.5, seed=61)
