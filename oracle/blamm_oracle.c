@@ -25,9 +25,16 @@
  *                         background) and the store-not-add bin quirk
  *                         (motif.h:109-115) on a constructed motif
  *   dict writer           closed-form ACGT counts of hand-built FASTA
+ *   empirical_hist        closed forms: L = 1 (bin counts = base counts),
+ *                         dyadic PWMs whose sums and bin edges are exact
+ *                         (pure-Python brute force), total = number of valid
+ *                         windows, lowercase-only windows, a window whose FP32
+ *                         left-to-right sum and exact sum fall on either side
+ *                         of a bin edge, width 0 (every window in bin 0)
  * Planted mistakes: compiling with -DORACLE_MUTATE_<name> swaps one rule for a
  * plausible wrong one (GT, NO_BG_SWAP, LOWER_COUNTS, NO_RUN_CHECK,
- * CUTOFF_DOUBLE, HIST_ADD, RUNS_IGNORE_RECORD); tests/test_oracle_mutations.py
+ * CUTOFF_DOUBLE, HIST_ADD, RUNS_IGNORE_RECORD, HIST_ROUND, HIST_NO_RUN_CHECK,
+ * HIST_FP64); tests/test_oracle_mutations.py
  * shows each is caught by a pinning check.
  * The reference itself is not built here: it needs the CMake-generated
  * config.h and an external BLAS (see the design notes), so no run of the
@@ -40,6 +47,7 @@
 #include <dlfcn.h>
 #include <errno.h>
 #include <glob.h>
+#include <limits.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdarg.h>
@@ -757,6 +765,110 @@ int64_t oracle_scan_fp64(const char *stream, uint64_t n, const uint64_t *runStar
     return (int64_t)total;
 }
 
+/* ---- empirical histograms (hist -e) -------------------------------------- */
+
+/* int(v) for a float as the reference's x86-64 hosts execute it (cvttss2si):
+ * truncation toward zero, and INT_MIN for NaN or a value outside the int
+ * range (ScoreHistogram::addObservation, motif.h:96-102). */
+static int trunc_x86(float v)
+{
+    if (v >= -2147483648.0f && v < 2147483648.0f) return (int)v;
+    return INT_MIN;
+}
+
+/* Bin of one observation (motif.h:96-102) with width = (max - min) / numBins
+ * in float (motif.h:58-66). */
+static int hist_bin(float score, float minScore, float width, size_t numBins)
+{
+#ifdef ORACLE_MUTATE_HIST_ROUND
+    int b = (int)lrintf((score - minScore) / width);
+#else
+    int b = trunc_x86((score - minScore) / width);
+#endif
+    if (b < 0) b = 0;
+    if (b > (int)numBins - 1) b = (int)numBins - 1;
+    return b;
+}
+
+typedef struct {
+    const char *stream; uint64_t n, qend;
+    const uint64_t *runStart; uint64_t nruns;
+    const float *pwm; int ldw; const int *len;
+    const float *minScore, *maxScore;
+    size_t numBins;
+    uint64_t *counts;
+    int c0, c1;
+} ehist_job;
+
+/* Histogram::extractObsScore (hist.cpp:70-91) over every offset of every
+ * block (hist.cpp:131-136): each window that satisfies the run rule
+ * (hist.cpp:83-86) adds its score to its motif's histogram.  The score is the
+ * FP32 sum of the window's one-hot row times the PWM column, accumulated in
+ * position order (the k order of an sgemm micro-kernel that keeps one
+ * accumulator per output; the reference's BLAS summation order is not fixed
+ * by the reference itself).  Lowercase bases contribute nothing. */
+static void *ehist_worker(void *arg)
+{
+    ehist_job *j = arg;
+    const uint64_t qe = j->qend < j->n ? j->qend : j->n;
+    for (int c = j->c0; c < j->c1; c++) {
+        const int L = j->len[c];
+        const float *w = j->pwm + (size_t)c * (size_t)j->ldw * 4;
+        const float width = (j->maxScore[c] - j->minScore[c]) / (float)j->numBins;
+        uint64_t *cnt = j->counts + (size_t)c * j->numBins;
+        uint64_t ri = 0;                                  /* first run start > q */
+        for (uint64_t q = 0; q < qe; q++) {
+            while (ri < j->nruns && j->runStart[ri] <= q) ri++;
+            const uint64_t nb = ri < j->nruns ? j->runStart[ri] : j->n;
+#ifdef ORACLE_MUTATE_HIST_NO_RUN_CHECK
+            if ((uint64_t)L > j->n - q) continue;
+#else
+            if ((uint64_t)L > nb - q) continue;
+#endif
+#ifdef ORACLE_MUTATE_HIST_FP64
+            double d = 0.0;
+            for (int p = 0; p < L; p++) {
+                int b = base_index((unsigned char)j->stream[q + (uint64_t)p]);
+                if (b < 4) d += (double)w[4 * p + b];
+            }
+            const float s = (float)d;
+#else
+            float s = 0.0f;
+            for (int p = 0; p < L; p++) {
+                int b = base_index((unsigned char)j->stream[q + (uint64_t)p]);
+                if (b < 4) s += w[4 * p + b];
+            }
+#endif
+            cnt[hist_bin(s, j->minScore[c], width, j->numBins)]++;
+        }
+    }
+    return NULL;
+}
+
+int oracle_empirical_hist(const char *stream, uint64_t n, const uint64_t *runStart,
+                          uint64_t nruns, const float *pwm, int ldw, const int *len,
+                          const float *minScore, const float *maxScore, int ncols,
+                          size_t numBins, uint64_t qend, int nthreads, uint64_t *counts)
+{
+    if (numBins < 1 || ncols < 0) return -1;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > ncols) nthreads = ncols > 0 ? ncols : 1;
+    ehist_job *jobs = xmalloc((size_t)nthreads * sizeof(ehist_job));
+    pthread_t *th = xmalloc((size_t)nthreads * sizeof(pthread_t));
+    for (int t = 0; t < nthreads; t++) {
+        ehist_job *j = &jobs[t];
+        j->stream = stream; j->n = n; j->qend = qend; j->runStart = runStart; j->nruns = nruns;
+        j->pwm = pwm; j->ldw = ldw; j->len = len; j->minScore = minScore; j->maxScore = maxScore;
+        j->numBins = numBins; j->counts = counts;
+        j->c0 = (int)((int64_t)ncols * t / nthreads);
+        j->c1 = (int)((int64_t)ncols * (t + 1) / nthreads);
+        pthread_create(&th[t], NULL, ehist_worker, j);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(jobs); free(th);
+    return 0;
+}
+
 /* ---- sgemm path ---------------------------------------------------------- */
 
 typedef void (*sgemm_fn)(const char *, const char *, const int *, const int *, const int *,
@@ -1103,6 +1215,7 @@ static int cmd_hist(int argc, char **argv)
     if (argc < 4) die("Usage: blamm_oracle hist [options] motifs.input sequences.input");
     size_t maxLength = 10000000, numBins = 250;
     char histdir[4096] = "";
+    int empirical = 0;
     for (int i = 2; i < argc - 2; i++) {   /* hist.cpp:187-215 */
         const char *a = argv[i];
         if ((!strcmp(a, "-l") || !strcmp(a, "--length")) && i + 1 < argc - 2) maxLength = strtoull(argv[++i], NULL, 10);
@@ -1112,7 +1225,7 @@ static int cmd_hist(int argc, char **argv)
             snprintf(histdir, sizeof(histdir), "%s", argv[++i]);
             size_t l = strlen(histdir);
             if (l == 0 || histdir[l - 1] != '/') strcat(histdir, "/");
-        } else if (!strcmp(a, "-e") || !strcmp(a, "--empirical")) die("oracle: empirical histograms are not restated");
+        } else if (!strcmp(a, "-e") || !strcmp(a, "--empirical")) empirical = 1;
         else die("Usage: blamm_oracle hist [options] motifs.input sequences.input");
     }
     settings_t st; settings_load(&st);
@@ -1121,16 +1234,50 @@ static int cmd_hist(int argc, char **argv)
     size_t ns; species_t *sp = load_dict(dictfn, &ns);
     size_t nm; motif_t *m = load_motifs(argv[argc - 2], 0, &nm);
     uint64_t *counts = xmalloc(numBins * sizeof(uint64_t));
+    int Lmax = 1;
+    for (size_t i = 0; i < nm; i++) if (m[i].L > Lmax) Lmax = m[i].L;
     for (size_t s = 0; s < ns; s++) {
         unsigned long long bsum = sp[s].counts[0] + sp[s].counts[1] + sp[s].counts[2] + sp[s].counts[3];
         float bgTot = (float)bsum; bgTot += 4.0f * st.pseudocount;
         float bg[4];
         for (int k = 0; k < 4; k++) bg[k] = ((float)sp[s].counts[k] + st.pseudocount) / bgTot;
+        /* empirical: every motif's histogram over the group's first maxLength
+         * valid bases (hist.cpp:140-160) */
+        uint64_t *ecounts = NULL;
+        float *emin = NULL, *emax = NULL;
+        if (empirical) {
+            float *P = xmalloc((nm ? nm : 1) * (size_t)Lmax * 4 * sizeof(float));
+            int *len = xmalloc((nm ? nm : 1) * sizeof(int));
+            emin = xmalloc((nm ? nm : 1) * sizeof(float));
+            emax = xmalloc((nm ? nm : 1) * sizeof(float));
+            memset(P, 0, (nm ? nm : 1) * (size_t)Lmax * 4 * sizeof(float));
+            for (size_t i = 0; i < nm; i++) {
+                float *w = P + i * (size_t)Lmax * 4;
+                oracle_pfm2pwm(m[i].pfm, m[i].L, sp[s].counts, st.pseudocount, 0, w);
+                len[i] = m[i].L;
+                emin[i] = oracle_min_score(w, m[i].L);
+                emax[i] = oracle_max_score(w, m[i].L);
+            }
+            char *stream; uint64_t *rs, *rq, *rp, nr, nseq; char err[1024];
+            int64_t n = oracle_filter_fasta((const char *const *)sp[s].files, (int)sp[s].nfiles, maxLength,
+                                            &stream, &rs, &rq, &rp, &nr, &nseq, err, sizeof(err));
+            if (n < 0) die("%s", err);
+            ecounts = xmalloc((nm ? nm : 1) * numBins * sizeof(uint64_t));
+            memset(ecounts, 0, (nm ? nm : 1) * numBins * sizeof(uint64_t));
+            oracle_empirical_hist(stream, (uint64_t)n, rs, nr, P, Lmax, len, emin, emax, (int)nm, numBins,
+                                  (uint64_t)n, (int)sysconf(_SC_NPROCESSORS_ONLN), ecounts);
+            free(stream); free(rs); free(rq); free(rp); free(P); free(len);
+        }
         for (size_t i = 0; i < nm; i++) {
             float *pwm = xmalloc((size_t)m[i].L * 4 * sizeof(float));
             oracle_pfm2pwm(m[i].pfm, m[i].L, sp[s].counts, st.pseudocount, 0, pwm);
             float mn, mx;
-            oracle_theoretical_hist(pwm, m[i].L, bg, numBins, maxLength, counts, &mn, &mx);
+            if (empirical) {
+                mn = emin[i]; mx = emax[i];
+                memcpy(counts, ecounts + i * numBins, numBins * sizeof(uint64_t));
+            } else {
+                oracle_theoretical_hist(pwm, m[i].L, bg, numBins, maxLength, counts, &mn, &mx);
+            }
             char fn[8192];
             snprintf(fn, sizeof(fn), "%shist_%s_%s.dat", histdir, sp[s].name, m[i].name);
             FILE *o = fopen(fn, "wb");
@@ -1142,6 +1289,7 @@ static int cmd_hist(int argc, char **argv)
             fclose(o);
             free(pwm);
         }
+        free(ecounts); free(emin); free(emax);
     }
     free(counts);
     return 0;

@@ -89,6 +89,17 @@ int oracle_theoretical_hist(const float *pwm, int L, const float *bgProb,
                             size_t numBins, size_t maxLength, uint64_t *counts,
                             float *minScore, float *maxScore);
 
+/* Empirical histograms (hist -e; hist.cpp:70-138, motif.h:58-66, 96-102):
+ * for each column c and every window q < min(qend, n) that satisfies the run
+ * rule, the FP32 left-to-right score (lowercase bases contribute 0) is binned
+ * as clamp(int((score - minScore[c]) / width), 0, numBins - 1) with
+ * width = (maxScore[c] - minScore[c]) / numBins in float and the x86 int
+ * conversion; counts[c * numBins + bin] is incremented (added to). */
+int oracle_empirical_hist(const char *stream, uint64_t n, const uint64_t *runStart,
+                          uint64_t nruns, const float *pwm, int ldw, const int *len,
+                          const float *minScore, const float *maxScore, int ncols,
+                          size_t numBins, uint64_t qend, int nthreads, uint64_t *counts);
+
 /* Command line entry (dict / hist / scan), same syntax as the reference. */
 int oracle_main(int argc, char **argv);
 

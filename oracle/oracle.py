@@ -76,6 +76,9 @@ class Oracle:
         L.oracle_free.argtypes = [ctypes.c_void_p]
         L.oracle_theoretical_hist.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_size_t, ctypes.c_size_t,
                                               _u64p, _f32p, _f32p]
+        L.oracle_empirical_hist.argtypes = [ctypes.c_char_p, ctypes.c_uint64, _u64p, ctypes.c_uint64, _f32p,
+                                            ctypes.c_int, _i32p, _f32p, _f32p, ctypes.c_int, ctypes.c_size_t,
+                                            ctypes.c_uint64, ctypes.c_int, _u64p]
         self.L = L
 
     def log2f(self, x: float) -> float:
@@ -115,6 +118,23 @@ class Oracle:
         self.L.oracle_theoretical_hist(_p(w, ctypes.c_float), w.shape[0], _p(bg, ctypes.c_float), num_bins,
                                        max_length, _p(counts, ctypes.c_uint64), ctypes.byref(mn), ctypes.byref(mx))
         return counts, mn.value, mx.value
+
+    def empirical_hist(self, stream: bytes, run_starts, pwms, min_scores, max_scores, num_bins: int = 250,
+                       qend: Optional[int] = None, threads: int = 8) -> np.ndarray:
+        """Empirical histograms (hist -e) of a filtered stream: (ncols, num_bins) uint64 counts."""
+        P, lens, lmax = _stack(pwms)
+        mn = np.ascontiguousarray(min_scores, np.float32)
+        mx = np.ascontiguousarray(max_scores, np.float32)
+        rs = np.ascontiguousarray(run_starts if run_starts is not None and len(run_starts) else [0], np.uint64)
+        counts = np.zeros((len(pwms), num_bins), np.uint64)
+        rc = self.L.oracle_empirical_hist(stream, len(stream), _p(rs, ctypes.c_uint64), len(rs),
+                                          _p(P, ctypes.c_float), lmax, _p(lens, ctypes.c_int),
+                                          _p(mn, ctypes.c_float), _p(mx, ctypes.c_float), len(pwms), num_bins,
+                                          len(stream) if qend is None else qend, threads,
+                                          _p(counts, ctypes.c_uint64))
+        if rc != 0:
+            raise RuntimeError("oracle empirical histogram failed")
+        return counts
 
     def filter_fasta(self, files, max_len: int = (1 << 64) - 1):
         """(stream bytes, run starts, run records, run positions, record count)."""
@@ -221,6 +241,10 @@ def min_score(w):
 
 def score_cutoff(*a, **k):
     return default().score_cutoff(*a, **k)
+
+
+def empirical_hist(*a, **k):
+    return default().empirical_hist(*a, **k)
 
 
 def theoretical_hist(*a, **k):

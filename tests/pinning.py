@@ -340,3 +340,112 @@ def check_scan_lowercase_zero(orc) -> None:
     assert set(hits["pos"].tolist()) == {0}      # 1 + 0 + 3 = 4 >= 4
     hits = orc.scan_fp64(text, [0], [w], np.array([4.5], np.float32), threads=1)
     assert len(hits) == 0                        # would be 6 if c counted
+
+
+# ---- empirical histograms (hist -e; hist.cpp:70-138, motif.h:58-66, 96-102) ------
+
+def _valid_windows(n, run_starts, L, qend=None):
+    """Window starts q whose L bases end at or before the next run start."""
+    starts = sorted(set(int(x) for x in run_starts) | {0}) + [n]
+    out = []
+    for a, b in zip(starts[:-1], starts[1:]):
+        out.extend(range(a, max(a, b - L + 1)))
+    return [q for q in out if qend is None or q < qend]
+
+
+def check_ehist_single_base(orc) -> None:
+    """L = 1: every base is a window; uppercase bases land in the bin of their
+    weight, lowercase ones in the bin of 0 (closed form from base counts)."""
+    rng = np.random.default_rng(31)
+    n = 5000
+    codes = rng.integers(0, 4, size=n)
+    text = bytearray(np.frombuffer(b"ACGT", np.uint8)[codes].tobytes())
+    low = rng.choice(n, size=400, replace=False)
+    for i in low:
+        text[i] |= 0x20
+    w = np.array([[-3.0, -1.0, 1.0, 3.0]], np.float32)
+    nb = 8                                        # width 1 on [-4, 4]: bins 1, 3, 5, 7; zero -> bin 4
+    got = orc.empirical_hist(bytes(text), [0, 17, 4000], [w], [-4.0], [4.0], nb, threads=1)[0]
+    upper = np.ones(n, bool)
+    upper[low] = False
+    want = np.zeros(nb, np.uint64)
+    for b, bin_ in zip(range(4), (1, 3, 5, 7)):
+        want[bin_] = np.count_nonzero((codes == b) & upper)
+    want[4] = len(low)
+    assert got.tolist() == want.tolist(), (got, want)
+
+
+def check_ehist_dyadic_bruteforce(orc) -> None:
+    """Weights on a 1/8 grid and a histogram of width 1/2: every sum, bin
+    coordinate and edge is exact (coordinates fall on quarters, so truncation
+    and rounding differ), and the bins follow from exact rational arithmetic
+    (pure Python).  Includes N-run breaks, lowercase bases and bins beyond the
+    histogram range on both sides (clamped)."""
+    idx = {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}
+    for seed in range(3):
+        rng = np.random.default_rng(40 + seed)
+        n = 1200
+        codes = rng.integers(0, 4, size=n)
+        text = bytearray(np.frombuffer(b"ACGT", np.uint8)[codes].tobytes())
+        for i in rng.choice(n, size=n // 8, replace=False):
+            text[i] |= 0x20
+        text = bytes(text)
+        runs = np.unique(np.concatenate([[0], rng.choice(n, size=10, replace=False)])).astype(np.uint64)
+        pwms = [(rng.integers(-16, 17, size=(int(rng.integers(1, 11)), 4)) / 8.0).astype(np.float32)
+                for _ in range(5)]
+        # ranges narrower than the score range so both clamps occur
+        lo = [Fraction(int(rng.integers(-24, -4)), 8) for _ in pwms]
+        nbs = [int(rng.integers(5, 30)) for _ in pwms]
+        his = [lo[c] + Fraction(nbs[c], 2) for c in range(len(pwms))]
+        qend = n - 37
+        for c, w in enumerate(pwms):
+            got = orc.empirical_hist(text, runs, [w], [float(lo[c])], [float(his[c])], nbs[c], qend=qend,
+                                     threads=1)[0]
+            want = [0] * nbs[c]
+            L = w.shape[0]
+            for q in _valid_windows(n, runs, L, qend):
+                s = Fraction(0)
+                for p in range(L):
+                    b = idx.get(text[q + p])
+                    if b is not None:
+                        s += Fraction(float(w[p, b]))
+                k = math.floor((s - lo[c]) * 2)     # int() truncates; negative values clamp to 0 anyway
+                want[min(max(k, 0), nbs[c] - 1)] += 1
+            assert got.tolist() == want, (seed, c)
+
+
+def check_ehist_total_windows(orc) -> None:
+    """Each histogram holds exactly one count per valid window: the sum over
+    runs of max(0, run length - L + 1), for windows starting before qend."""
+    rng = np.random.default_rng(52)
+    n = 30_000
+    codes = rng.integers(0, 4, size=n)
+    text = np.frombuffer(b"ACGT", np.uint8)[codes].tobytes()
+    runs = np.unique(np.concatenate([[0], rng.choice(n, size=300, replace=False),
+                                     np.arange(1000, 1030)])).astype(np.uint64)
+    pwms = [rng.normal(0, 1.5, size=(L, 4)).astype(np.float32) for L in (1, 2, 7, 16, 30)]
+    mn = [orc.min_score(w) for w in pwms]
+    mx = [orc.max_score(w) for w in pwms]
+    for qend in (n, 12_345):
+        got = orc.empirical_hist(text, runs, pwms, mn, mx, 250, qend=qend, threads=2)
+        for c, w in enumerate(pwms):
+            assert int(got[c].sum()) == len(_valid_windows(n, runs, w.shape[0], qend)), (qend, c)
+
+
+def check_ehist_fp32_order(orc) -> None:
+    """The score binned is the FP32 sum in position order: for weights
+    1 - 2^-24, 2^-26, 2^-26, 2^-26, 2^-26 each FP32 add rounds back to
+    1 - 2^-24 (bin 0 of width 1), while the exact sum is 1.0 (bin 1)."""
+    w = np.full((5, 4), 0.0, np.float32)
+    w[0, 0] = np.float32(1.0) - np.float32(2.0 ** -24)
+    w[1:, 0] = np.float32(2.0 ** -26)
+    got = orc.empirical_hist(b"AAAAA", [0], [w], [0.0], [2.0], 2, threads=1)[0]
+    assert got.tolist() == [1, 0], got
+
+
+def check_ehist_width_zero(orc) -> None:
+    """min == max gives width 0: (score - min) / 0 is +-inf or NaN, which the
+    x86 int conversion turns into INT_MIN, clamped to bin 0."""
+    w = np.array([[0.5, -0.5, 0.25, 0.0]] * 3, np.float32)
+    got = orc.empirical_hist(b"ACGTTGCAAC", [0], [w], [0.25], [0.25], 10, threads=1)[0]
+    assert got.tolist() == [8] + [0] * 9, got

@@ -14,6 +14,9 @@ MUTATIONS = {
     "ORACLE_MUTATE_CUTOFF_DOUBLE": "check_cutoff_float_coordinate",
     "ORACLE_MUTATE_HIST_ADD": "check_hist_store_not_add",
     "ORACLE_MUTATE_RUNS_IGNORE_RECORD": "check_fasta_cases",
+    "ORACLE_MUTATE_HIST_ROUND": "check_ehist_dyadic_bruteforce",
+    "ORACLE_MUTATE_HIST_NO_RUN_CHECK": "check_ehist_total_windows",
+    "ORACLE_MUTATE_HIST_FP64": "check_ehist_fp32_order",
 }
 
 
