@@ -156,6 +156,10 @@ def hip_lib() -> ctypes.CDLL:
         lib.blamm_hip_server_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         lib.blamm_hip_server_release.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(RequestInfo)]
         lib.blamm_hip_server_destroy.argtypes = [vp]
+        lib.blamm_hip_hist.argtypes = [vp, _u64p, _u64p, ctypes.c_uint64, _u64p, ctypes.c_uint64, ctypes.c_uint64,
+                                       _f32p, _f32p, ctypes.c_uint32, _u64p]
+        lib.blamm_hip_hist_device.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint64, ctypes.c_uint64, vp,
+                                              ctypes.c_uint64, ctypes.c_uint64, _f32p, _f32p, ctypes.c_uint32, _u64p]
         _hip = lib
     return _hip
 
@@ -200,6 +204,18 @@ def pwm(pfm: np.ndarray, bg_counts, pseudocount: float = 0.25, rc: bool = False)
                                  ctypes.c_float(pseudocount), int(rc), _ptr(out, ctypes.c_float)) != 0:
         raise RuntimeError(host_lib().blamm_host_last_error().decode())
     return out
+
+
+def max_score(w: np.ndarray) -> float:
+    """Motif::getMaxScore of a PWM (L x 4 float32), as `blamm` computes it."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    return float(host_lib().blamm_host_max_score(_ptr(w, ctypes.c_float), w.shape[0]))
+
+
+def min_score(w: np.ndarray) -> float:
+    """Motif::getMinScore of a PWM (L x 4 float32), as `blamm` computes it."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    return float(host_lib().blamm_host_min_score(_ptr(w, ctypes.c_float), w.shape[0]))
 
 
 def pvalue_threshold(w: np.ndarray, bg_counts, pvalue: float, pseudocount: float = 0.25,
@@ -333,6 +349,42 @@ class HipScanner:
                                               nbases, _ptr(runs, ctypes.c_uint64) if len(runs) else None,
                                               len(runs), ctypes.byref(p), ctypes.byref(n)))
         return _hits_array(p.value or 0, int(n.value))
+
+    def hist(self, codes: np.ndarray, nbases: int, min_scores, max_scores, num_bins: int = 250, run_starts=None,
+             lower: Optional[np.ndarray] = None, qend: Optional[int] = None) -> np.ndarray:
+        """Empirical histograms (hist -e) of a host stream for every library
+        column: (ncols, num_bins) uint64 counts."""
+        pw = int(self.lib.blamm_hip_packed_words(nbases))
+        cbuf = np.zeros(pw, dtype=np.uint64)
+        cbuf[:min(len(codes), pw)] = codes[:pw]
+        mbuf = None
+        if lower is not None:
+            mw = int(self.lib.blamm_hip_mask_words(nbases))
+            mbuf = np.zeros(mw, dtype=np.uint64)
+            mbuf[:min(len(lower), mw)] = lower[:mw]
+        runs = np.ascontiguousarray(run_starts if run_starts is not None else [], dtype=np.uint64)
+        mn = np.ascontiguousarray(min_scores, np.float32)
+        mx = np.ascontiguousarray(max_scores, np.float32)
+        counts = np.zeros((len(mn), num_bins), np.uint64)
+        _check(self.lib.blamm_hip_hist(self.ctx, _ptr(cbuf, ctypes.c_uint64),
+                                       _ptr(mbuf, ctypes.c_uint64) if mbuf is not None else None, nbases,
+                                       _ptr(runs, ctypes.c_uint64) if len(runs) else None, len(runs),
+                                       nbases if qend is None else qend, _ptr(mn, ctypes.c_float),
+                                       _ptr(mx, ctypes.c_float), num_bins, _ptr(counts, ctypes.c_uint64)))
+        return counts
+
+    def hist_device(self, d_codes, d_lower, nbases: int, d_runs, nruns: int, min_scores, max_scores,
+                    num_bins: int = 250, qend: Optional[int] = None) -> np.ndarray:
+        """hist() on device-resident buffers (torch tensors)."""
+        mn = np.ascontiguousarray(min_scores, np.float32)
+        mx = np.ascontiguousarray(max_scores, np.float32)
+        counts = np.zeros((len(mn), num_bins), np.uint64)
+        _check(self.lib.blamm_hip_hist_device(
+            self.ctx, d_codes.data_ptr(), d_codes.numel(),
+            d_lower.data_ptr() if d_lower is not None else None, d_lower.numel() if d_lower is not None else 0,
+            nbases, d_runs.data_ptr() if nruns else None, nruns, nbases if qend is None else qend,
+            _ptr(mn, ctypes.c_float), _ptr(mx, ctypes.c_float), num_bins, _ptr(counts, ctypes.c_uint64)))
+        return counts
 
     def stats(self) -> ScanStats:
         st = ScanStats()

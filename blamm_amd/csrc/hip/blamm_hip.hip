@@ -20,6 +20,7 @@
 
 #include "../../../include/blamm_hip.h"
 #include "scan_kernels.hpp"
+#include "hist_kernels.hpp"
 
 using namespace blamm;
 
@@ -150,6 +151,11 @@ struct blamm_hip_ctx {
     DevBuf<double> slot_band_hi, slot_band_hi_any;      // hi-only stage of the bf16 filter
     DevBuf<int> slot_len;
     DevBuf<uint32_t> slot_col, groups_d;
+
+    // empirical histograms
+    DevBuf<HistSlot> hist_slots;
+    DevBuf<uint32_t> hist_groups;
+    DevBuf<unsigned long long> hist_counts;
 
     // scratch
     DevBuf<uint4> cand;
@@ -555,6 +561,7 @@ int blamm_hip_destroy(blamm_hip_ctx* c)
     c->packed.release(); c->lcmask.release(); c->runs.release(); c->nhits_dev.release();
     c->hits.release();
     c->hits_pinned.release();
+    c->hist_slots.release(); c->hist_groups.release(); c->hist_counts.release();
     for (int i = 0; i < 4; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -741,10 +748,10 @@ int blamm_hip_last_stats(blamm_hip_ctx* c, blamm_scan_stats* st)
 
 namespace {
 
-// Upload a host stream into the context's staging buffers and run the scan
-// into c->hits (grown until every hit fits); *nhits = hits on the device.
-int scan_staged(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
-                const uint64_t* run_starts, uint64_t nruns, uint64_t cap_limit, uint64_t* nhits)
+// Upload a host stream into the context's staging buffers (c->packed,
+// c->lcmask, c->runs), on the context's stream.
+int stage_stream(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
+                 const uint64_t* run_starts, uint64_t nruns)
 {
     const uint64_t pw = blamm_hip_packed_words(nbases), mw = blamm_hip_mask_words(nbases);
     HIP_TRY(c->packed.reserve(pw));
@@ -759,6 +766,16 @@ int scan_staged(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask
     HIP_TRY(c->runs.reserve(nruns + 1));
     if (nruns > 0)
         HIP_TRY(hipMemcpyAsync(c->runs.p, run_starts, nruns * 8, hipMemcpyHostToDevice, c->stream));
+    return BLAMM_OK;
+}
+
+// Upload a host stream and run the scan into c->hits (grown until every hit
+// fits); *nhits = hits on the device.
+int scan_staged(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
+                const uint64_t* run_starts, uint64_t nruns, uint64_t cap_limit, uint64_t* nhits)
+{
+    BLAMM_TRY(stage_stream(c, packed, lcmask, nbases, run_starts, nruns));
+    const uint64_t pw = blamm_hip_packed_words(nbases), mw = blamm_hip_mask_words(nbases);
     uint64_t dcap = std::min<uint64_t>(cap_limit, std::max<uint64_t>(c->hits.n, 1u << 16));
     for (int attempt = 0; attempt < 8; ++attempt) {
         HIP_TRY(c->hits.reserve(std::max<uint64_t>(dcap, 1)));
@@ -833,3 +850,4 @@ int blamm_hip_scan(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcm
 #include "batch.hpp"
 #include "tune.hpp"
 #include "serve.hpp"
+#include "hist.hpp"

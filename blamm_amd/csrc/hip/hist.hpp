@@ -1,0 +1,219 @@
+// Empirical score histograms (`blamm hist -e`) through the C ABI: replaces
+// Histogram::histThread's sgemm and extractObsScore loop (hist.cpp:70-138)
+// for one stream of a manifest group.  Included at the end of blamm_hip.hip.
+
+namespace {
+
+// Bin parameters of every library slot for histograms on [mins[col],
+// maxs[col]] with nbins bins (kernels in hist_kernels.hpp).
+void hist_slot_params(const blamm_hip_ctx* c, const float* mins, const float* maxs, uint32_t nbins,
+                      std::vector<HistSlot>& hs)
+{
+    hs.assign((size_t)c->nslots, HistSlot{0.f, 0.f, 2.f, -1.f, 0.f, 0.f, 0u, 0u});
+    for (int s = 0; s < c->nslots; ++s) {
+        const int col = c->slot_col_h[s];
+        if (col < 0) continue;
+        HistSlot h{0.f, 0.f, 2.f, -1.f, 0.f, 0.f, 0u, 0u};   // mlo > mhi: every window binned exactly
+        const int L = c->lib_len[col];
+        const float mn = mins[col], mx = maxs[col];
+        const float wd = (mx - mn) / (float)nbins;          // motif.h:64
+        h.mn = mn;
+        h.wd = wd;
+        h.len = (uint32_t)L;
+        const float* w = c->lib_P.data() + (size_t)col * c->ldp;
+        double smax = 0.0;   // bound on |score| of any window
+        for (int p = 0; p < L; ++p) {
+            double m = 0.0;
+            for (int b = 0; b < 4; ++b) m = std::max(m, (double)std::fabs(w[4 * p + b]));
+            smax += m;
+        }
+        // |MFMA score - FP32 left-to-right score|: half the filter band bounds
+        // the MFMA side against the exact sum, half the sequential band the
+        // FP32 side
+        const double ds = 0.5 * (c->band_h[s] + c->seq_h[s]);
+        if (std::isfinite(mn) && std::isfinite(mx) && wd == 0.f) {
+            // (score - min) / 0 is +-inf or NaN for every score; int() of
+            // those is INT_MIN, clamped to bin 0
+            h.iw = 0.f;
+            h.c = 0.5f;
+            h.mlo = 0.f;
+            h.mhi = 1.f;
+        } else if (std::isfinite(mn) && std::isfinite(wd) && wd > 0.f) {
+            const float iw = 1.0f / wd;
+            const float cf = -(mn * iw);
+            const double inv = 1.0 / (double)wd;
+            const double S = smax * (1.0 + 0x1p-20);
+            // largest |bin coordinate| of any score, and the margin covering
+            // |x - the reference's coordinate of the FP32 score|: the score
+            // difference, the rounding of iw and c, of the fma, of the
+            // reference's subtract and divide, and of fract
+            const double qmax = (S + std::fabs((double)mn) + ds) * inv + 1.0;
+            const double m = ds * inv * (1.0 + 0x1p-20) + (S + ds + 2.0 * std::fabs((double)mn)) * inv * 0x1p-22 +
+                             (qmax + 1.0) * 0x1p-21 + 0x1p-22;
+            if (std::isfinite(iw) && std::isfinite(cf) && qmax < 0x1p30 && m < 0.25) {
+                float lo = (float)m;
+                if ((double)lo < m) lo = std::nextafter(lo, 1.f);
+                float hi = (float)(1.0 - m);
+                if ((double)hi > 1.0 - m) hi = std::nextafter(hi, 0.f);
+                h.iw = iw;
+                h.c = cf;
+                h.mlo = lo;
+                h.mhi = hi;
+            }
+        }
+        hs[(size_t)s] = h;
+    }
+}
+
+template <bool MASK, bool LDS>
+const void* hist_kernel_ptr()
+{
+    return (const void*)hist_filter_kernel<MASK, LDS>;
+}
+
+}  // namespace
+
+extern "C" {
+
+int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t packed_words,
+                          const uint64_t* d_lcmask, uint64_t mask_words, uint64_t nbases,
+                          const uint64_t* d_run_starts, uint64_t nruns, uint64_t qend, const float* min_score,
+                          const float* max_score, uint32_t nbins, uint64_t* counts)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "hist: null context");
+    if (!c->have_lib) return fail(BLAMM_ERR_STATE, "hist: no library uploaded");
+    if (c->pending) return fail(BLAMM_ERR_STATE, "hist: a scan is in flight");
+    if (c->precision != BLAMM_PREC_BF16X2)
+        return fail(BLAMM_ERR_ARG, "hist: the library must be uploaded in the split-BF16 form");
+    if (!min_score || !max_score || !counts) return fail(BLAMM_ERR_ARG, "hist: null histogram range or counts");
+    if (nbins < 1 || nbins > (1u << 24)) return fail(BLAMM_ERR_ARG, "hist: nbins outside 1..2^24");
+    if (nbases >= (1ull << 32) - 512) return fail(BLAMM_ERR_ARG, "hist: nbases must be < 2^32 - 512 per call");
+    if (nbases > 0 && !d_packed) return fail(BLAMM_ERR_ARG, "hist: null packed stream");
+    if (nbases > 0 && packed_words < blamm_hip_packed_words(nbases))
+        return fail(BLAMM_ERR_ARG, "hist: packed buffer shorter than blamm_hip_packed_words(nbases)");
+    if (d_lcmask && mask_words < blamm_hip_mask_words(nbases))
+        return fail(BLAMM_ERR_ARG, "hist: mask buffer shorter than blamm_hip_mask_words(nbases)");
+    if (nruns > 0 && !d_run_starts) return fail(BLAMM_ERR_ARG, "hist: null run starts");
+    if (nruns >= (1ull << 32)) return fail(BLAMM_ERR_ARG, "hist: too many runs");
+    if ((uint64_t)c->nslots * nbins > (1ull << 34)) return fail(BLAMM_ERR_ARG, "hist: columns x bins too large");
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (qend > nbases) qend = nbases;
+
+    std::vector<HistSlot> hs;
+    hist_slot_params(c, min_score, max_score, nbins, hs);
+    // LDS histograms when a block's 32 columns fit; two blocks per item when
+    // both fit
+    const bool lds = (uint64_t)nbins * kBf16Cols * 4 <= kHistLdsBytes;
+    const uint32_t hnb = (lds && 2ull * nbins * kBf16Cols * 4 <= kHistLdsBytes) ? 2u : 1u;
+    std::vector<uint32_t> groups;
+    for (const GroupInfo& g : c->groups)
+        for (uint32_t i = 0; i < g.nb; i += hnb)
+            groups.push_back(group_pack(g.blk0 + i, g.nch, std::min(hnb, g.nb - i)));
+    HIP_TRY(c->hist_slots.reserve(hs.size()));
+    HIP_TRY(c->hist_groups.reserve(groups.size()));
+    const size_t ncount = (size_t)c->nslots * nbins;
+    HIP_TRY(c->hist_counts.reserve(ncount));
+    HIP_TRY(hipMemcpyAsync(c->hist_slots.p, hs.data(), hs.size() * sizeof(HistSlot), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->hist_groups.p, groups.data(), groups.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           s));
+    if (c->cand.n == 0) HIP_TRY(c->cand.reserve(1u << 20));
+
+    const bool mask = d_lcmask != nullptr;
+    const uint32_t dyn = lds ? hnb * kBf16Cols * nbins * 4 : 0u;
+    const void* kp = mask ? (lds ? hist_kernel_ptr<true, true>() : hist_kernel_ptr<true, false>())
+                          : (lds ? hist_kernel_ptr<false, true>() : hist_kernel_ptr<false, false>());
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, kThreads, dyn) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const uint32_t grid = (uint32_t)c->num_cu * (uint32_t)per_cu;
+
+    HistArgs a{};
+    a.packed = d_packed;
+    a.lcmask = d_lcmask;
+    a.frag_hi = (const uint4*)c->frag_a.p;
+    a.frag_lo = (const uint4*)c->frag_b.p;
+    a.hs = c->hist_slots.p;
+    a.slot_w = c->slot_w.p;
+    a.run_starts = d_run_starts;
+    a.groups = c->hist_groups.p;
+    a.counts = c->hist_counts.p;
+    a.counters = c->counters.p;
+    a.nbases = (uint32_t)nbases;
+    a.nruns = (uint32_t)nruns;
+    a.qend = (uint32_t)qend;
+    a.nbins = nbins;
+    a.ntiles = (uint32_t)((qend + kBf16Tile - 1) / kBf16Tile);   // windows start before qend
+    a.ngroups = (uint32_t)groups.size();
+    {
+        // about four items per workgroup, 16..1024 tiles per item
+        const uint64_t want = ((uint64_t)a.ntiles * a.ngroups + 4ull * grid - 1) / (4ull * grid);
+        a.chunk_tiles = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(16, want));
+        const uint64_t nchunks = (a.ntiles + a.chunk_tiles - 1) / a.chunk_tiles;
+        if (nchunks * a.ngroups >= (1ull << 32)) return fail(BLAMM_ERR_ARG, "hist: work list too long");
+        a.nitems = (uint32_t)(nchunks * a.ngroups);
+    }
+
+    unsigned long long cnt[kCntSlots];
+    for (int attempt = 0;; ++attempt) {
+        if (attempt == 4) return fail(BLAMM_ERR_DEVICE, "hist: capacity retries exhausted");
+        a.cand = c->cand.p;
+        a.cand_cap = c->cand.n;
+        HIP_TRY(hipMemsetAsync(c->hist_counts.p, 0, ncount * sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, c->counters.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[0], s));
+        if (a.nitems > 0) {
+            if (mask) {
+                if (lds) hipLaunchKernelGGL((hist_filter_kernel<true, true>), dim3(grid), dim3(kThreads), dyn, s, a);
+                else hipLaunchKernelGGL((hist_filter_kernel<true, false>), dim3(grid), dim3(kThreads), dyn, s, a);
+            } else {
+                if (lds) hipLaunchKernelGGL((hist_filter_kernel<false, true>), dim3(grid), dim3(kThreads), dyn, s, a);
+                else hipLaunchKernelGGL((hist_filter_kernel<false, false>), dim3(grid), dim3(kThreads), dyn, s, a);
+            }
+            HIP_TRY(hipGetLastError());
+        }
+        HIP_TRY(hipEventRecord(c->ev[1], s));
+        hipLaunchKernelGGL(hist_exact_kernel, dim3((uint32_t)c->num_cu * 8), dim3(kThreads), 0, s, a);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[2], s));
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+        if (cnt[kCntCand] <= c->cand.n) break;
+        HIP_TRY(c->cand.reserve((size_t)(cnt[kCntCand] + cnt[kCntCand] / 4 + 1024)));
+    }
+    std::vector<unsigned long long> dev(ncount);
+    HIP_TRY(hipMemcpy(dev.data(), c->hist_counts.p, ncount * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int sl = 0; sl < c->nslots; ++sl) {
+        const int col = c->slot_col_h[sl];
+        if (col < 0) continue;
+        for (uint32_t b = 0; b < nbins; ++b)
+            counts[(size_t)col * nbins + b] += dev[(size_t)sl * nbins + b];
+    }
+    float ms_f = 0.f, ms_all = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms_f, c->ev[0], c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]));
+    c->stats = blamm_scan_stats{};
+    c->stats.candidates = cnt[kCntCand];
+    c->stats.filter_ms = ms_f;
+    c->stats.kernel_ms = ms_all;
+    return BLAMM_OK;
+}
+
+int blamm_hip_hist(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask, uint64_t nbases,
+                   const uint64_t* run_starts, uint64_t nruns, uint64_t qend, const float* min_score,
+                   const float* max_score, uint32_t nbins, uint64_t* counts)
+{
+    if (!c) return fail(BLAMM_ERR_ARG, "hist: null context");
+    if (nbases > 0 && !packed) return fail(BLAMM_ERR_ARG, "hist: null packed stream");
+    if (nruns > 0 && !run_starts) return fail(BLAMM_ERR_ARG, "hist: null run starts");
+    if (nbases >= (1ull << 32) - 512) return fail(BLAMM_ERR_ARG, "hist: nbases must be < 2^32 - 512 per call");
+    if (c->pending) return fail(BLAMM_ERR_STATE, "hist: a scan is in flight");
+    HIP_TRY(hipSetDevice(c->device));
+    BLAMM_TRY(stage_stream(c, packed, lcmask, nbases, run_starts, nruns));
+    return blamm_hip_hist_device(c, c->packed.p, blamm_hip_packed_words(nbases), lcmask ? c->lcmask.p : nullptr,
+                                 blamm_hip_mask_words(nbases), nbases, c->runs.p, nruns, qend, min_score,
+                                 max_score, nbins, counts);
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// Empirical score histograms (`blamm hist -e`) on the split-BF16 engine.
+//
+// The reference fills them with the scan's sgemm (Histogram::histThread,
+// hist.cpp:93-138) and bins the FP32 score of every window that satisfies the
+// run rule (Histogram::extractObsScore, hist.cpp:70-91) with
+// ScoreHistogram::addObservation (motif.h:96-102):
+//   bin = clamp(int((score - min) / width), 0, nbins - 1),
+//   width = (max - min) / nbins (motif.h:58-66), all in float.
+// The score binned here is the FP32 sum of the window's weights in position
+// order (lowercase bases contribute nothing).
+//
+// hist_filter_kernel scores windows with the same hi+lo MFMA tiles as the
+// scan and decides a window's bin from its MFMA score a when that is safe:
+// x = fma(a, 1/width, -min/width) approximates the bin coordinate of the FP32
+// score to within a per-column margin m (from the split and summation error
+// bounds and the rounding of every float step, computed on the host), so when
+// fract(x) lies in [m, 1 - m] the FP32 score's coordinate lies in the same
+// unit interval and the bin is trunc(x), clamped.  Counts go to a
+// workgroup-private LDS histogram that is added to HBM once per work item.
+// Windows near a bin edge (about 2m of them) are recorded, and
+// hist_exact_kernel bins their FP32 sum with the reference's arithmetic.
+#pragma once
+
+#include "scan_kernels.hpp"
+
+namespace blamm {
+
+struct HistSlot {
+    float iw, c;       // x = fma(score, iw, c)
+    float mlo, mhi;    // bin decided from x iff mlo <= fract(x) <= mhi
+    float mn, wd;      // histogram minimum and bin width, as the reference computes them
+    uint32_t len;      // motif length; 0 = empty slot
+    uint32_t pad;
+};
+
+struct HistArgs {
+    const uint64_t* packed;
+    const uint64_t* lcmask;        // MASK variants only
+    const uint4* frag_hi;
+    const uint4* frag_lo;
+    const HistSlot* hs;
+    const float* slot_w;           // [slot][kMaxLen][4] FP32 weights
+    const uint64_t* run_starts;
+    const uint32_t* groups;        // group_pack(blk0, nch, nb), nb <= kHistNbMax
+    unsigned long long* counts;    // [slot][nbins]
+    uint4* cand;                   // deferred windows: (first window of the lane, slot, rows, 0)
+    unsigned long long* counters;  // kCntCand: deferred records; kCntQueue: work item head
+    uint64_t cand_cap;
+    uint32_t nbases, nruns, qend, nbins;
+    uint32_t ntiles, chunk_tiles, ngroups, nitems;
+};
+
+constexpr int kHistNbMax = 2;
+// LDS histogram budget per workgroup (kHistNbMax blocks x 32 columns x 256 bins).
+constexpr uint32_t kHistLdsBytes = 64 * 1024;
+
+// ScoreHistogram::addObservation's bin (motif.h:96-102) of an FP32 score:
+// float subtract and IEEE division, then int() as the reference's x86-64
+// hosts execute it (truncation; NaN or out of range gives INT_MIN), clamped.
+__device__ __forceinline__ uint32_t hist_bin_ref(float s, float mn, float wd, uint32_t nbins)
+{
+    const float v = (s - mn) / wd;
+    int b = (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : (int)0x80000000u;
+    b = max(b, 0);
+    b = min(b, (int)nbins - 1);
+    return (uint32_t)b;
+}
+
+// Does the window at q of length len end at or before the first run start
+// after q?  rnext is the first run start after the tile's first window and jr
+// its index (nbases past the last run).
+__device__ __forceinline__ bool hist_row_fits(const HistArgs& a, uint32_t q, uint32_t len, uint32_t jr,
+                                              uint32_t rnext)
+{
+    uint32_t r = rnext;
+    if (q >= rnext) {
+        uint32_t k = jr + 1;
+        while (k < a.nruns && (uint32_t)a.run_starts[k] <= q) ++k;
+        r = k < a.nruns ? (uint32_t)a.run_starts[k] : a.nbases;
+    }
+    return q + len <= r;
+}
+
+// Tiles t0, t0 + 4, ... < t1 of column blocks blk0 .. blk0+NB-1 (the wave's
+// share of a work item).  hl: the workgroup's LDS histogram of the item's
+// columns (LDS variant).
+template <int NCH, int NB, bool MASK, bool LDS>
+__device__ __forceinline__ void hist_range(const HistArgs& a, Stage& st, uint32_t* hl, uint32_t blk0, uint32_t t0,
+                                           uint32_t t1, uint32_t lane)
+{
+    bf16x8 bh[NB][NCH], bl[NB][NCH];
+    float iw[NB], cc[NB], mlo[NB], mhi[NB];
+    uint32_t len[NB], slot[NB], hoff[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+#pragma unroll
+        for (int j = 0; j < NCH; ++j) {
+            const size_t idx = ((size_t)(blk0 + b) * kMaxChunks + j) * 64 + lane;
+            bh[b][j] = __builtin_bit_cast(bf16x8, a.frag_hi[idx]);
+            bl[b][j] = __builtin_bit_cast(bf16x8, a.frag_lo[idx]);
+        }
+        slot[b] = (blk0 + b) * kBf16Cols + (lane & 31);
+        const HistSlot h = a.hs[slot[b]];
+        iw[b] = h.iw;
+        cc[b] = h.c;
+        mlo[b] = h.mlo;
+        mhi[b] = h.mhi;
+        len[b] = h.len;
+        hoff[b] = LDS ? (uint32_t)(b * kBf16Cols + (lane & 31)) * a.nbins : 0u;
+    }
+    const uint32_t off = (lane & 31) + 2 * (lane >> 5);   // first base of the lane's window, 0..33
+    const uint32_t woff = off >> 5;
+    const uint32_t sh = 2 * (off & 31), sh_inv = 63 - sh;
+    const uint64_t* __restrict__ pk = a.packed + woff;
+    const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + woff;
+
+    // run cursor: first run start after the current tile's first window
+    uint32_t jr, rnext;
+    {
+        const uint32_t q = t0 * kBf16Tile;
+        uint32_t l = 0, h = a.nruns;
+        while (l < h) {
+            const uint32_t mid = (l + h) >> 1;
+            if ((uint32_t)a.run_starts[mid] <= q) l = mid + 1; else h = mid;
+        }
+        jr = l;
+        rnext = l < a.nruns ? (uint32_t)a.run_starts[l] : a.nbases;
+    }
+    const uint32_t last = a.nbins - 1;
+    for (uint32_t t = t0; t < t1; t += kWavesPerBlock) {
+        const uint32_t q0 = t * kBf16Tile;
+        const uint64_t bits = funnel64(pk[t], pk[t + 1], sh, sh_inv);
+        const uint32_t mb = MASK ? funnel32(mk[t], mk[t + 1], off & 31) : 0u;
+        f32x16 acc[NB];
+        bf16_mfma_tile<NCH, NB, MASK>(acc, bh, bl, bits, mb);
+        while (rnext <= q0) {
+            ++jr;
+            rnext = jr < a.nruns ? (uint32_t)a.run_starts[jr] : a.nbases;
+        }
+        // every window of the tile fits its run for every length <= 4 NCH
+        const bool fast = q0 + (kBf16Tile - 1) + 4 * NCH <= rnext && q0 + kBf16Tile <= a.qend;
+        mfma_result_guard();
+        const uint32_t qlane = q0 + 4 * (lane >> 5);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            uint32_t defer = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t q = qlane + 8 * (r >> 2) + (r & 3);
+                bool v = len[b] != 0;
+                if (!fast) v = v && q < a.qend && hist_row_fits(a, q, len[b], jr, rnext);
+                const float x = __builtin_fmaf(acc[b][r], iw[b], cc[b]);
+                const float fr = __builtin_amdgcn_fractf(x);
+                const bool dec = fr >= mlo[b] && fr <= mhi[b];
+                if (v && dec) {
+                    const uint32_t bin = (uint32_t)min(max((int)x, 0), (int)last);
+                    if (LDS) atomicAdd(&hl[hoff[b] + bin], 1u);
+                    else atomicAdd(&a.counts[(size_t)slot[b] * a.nbins + bin], 1ull);
+                }
+                defer |= (v && !dec ? 1u : 0u) << r;
+            }
+            const uint64_t m = __ballot(defer != 0u);
+            if (m) {
+                if (st.n > (uint32_t)(kStage - 64)) stage_flush(a, st);
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (defer) st.buf[st.n + rank] = make_uint4(qlane, slot[b], defer, 0u);
+                st.n += (uint32_t)__popcll(m);
+            }
+        }
+    }
+}
+
+// Persistent: each workgroup takes work items (chunk of chunk_tiles tiles,
+// column group) from one queue; its four waves share the item's tiles and
+// its LDS histogram.
+template <bool MASK, bool LDS>
+__global__ __launch_bounds__(kThreads) void hist_filter_kernel(HistArgs a)
+{
+    extern __shared__ uint32_t hist_lds[];
+    __shared__ uint4 stage_lds[kWavesPerBlock][kStage];
+    __shared__ uint32_t item_s;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    Stage st{stage_lds[wave], 0u, nullptr};
+    for (;;) {
+        if (threadIdx.x == 0) item_s = (uint32_t)atomicAdd(&a.counters[kCntQueue], 1ull);
+        __syncthreads();
+        const uint32_t it = item_s;
+        if (it >= a.nitems) break;
+        const uint32_t k = it / a.ngroups, g = it - k * a.ngroups;
+        const uint32_t gp = a.groups[g];
+        const uint32_t blk0 = gp & 0xFFFFFu, nch = (gp >> 20) & 63u, nb = gp >> 26;
+        const uint32_t t0 = k * a.chunk_tiles, t1 = min(t0 + a.chunk_tiles, a.ntiles);
+        const uint32_t used = nb * kBf16Cols * a.nbins;
+        if (LDS)
+            for (uint32_t i = threadIdx.x; i < used; i += kThreads) hist_lds[i] = 0u;
+        __syncthreads();
+        switch (nch * 8 + nb) {
+#define BLAMM_H(N, B) case N * 8 + B: hist_range<N, B, MASK, LDS>(a, st, hist_lds, blk0, t0 + wave, t1, lane); break;
+            BLAMM_H(1, 1) BLAMM_H(2, 1) BLAMM_H(3, 1) BLAMM_H(4, 1)
+            BLAMM_H(5, 1) BLAMM_H(6, 1) BLAMM_H(7, 1) BLAMM_H(8, 1)
+            BLAMM_H(1, 2) BLAMM_H(2, 2) BLAMM_H(3, 2) BLAMM_H(4, 2)
+            BLAMM_H(5, 2) BLAMM_H(6, 2) BLAMM_H(7, 2) BLAMM_H(8, 2)
+#undef BLAMM_H
+            default: break;
+        }
+        if (LDS) {
+            __syncthreads();
+            unsigned long long* dst = a.counts + (size_t)blk0 * kBf16Cols * a.nbins;
+            for (uint32_t i = threadIdx.x; i < used; i += kThreads) {
+                const uint32_t v = hist_lds[i];
+                if (v) atomicAdd(&dst[i], (unsigned long long)v);
+            }
+        }
+        // the next item's first barrier orders these reads before its zeroing
+    }
+    stage_flush(a, st);
+}
+
+// Exact bins of the deferred windows: FP32 left-to-right sum of the window's
+// weights (lowercase bases skipped) and the reference's float binning.
+// One thread per record; the run rule was applied when the record was made.
+__global__ __launch_bounds__(kThreads) void hist_exact_kernel(HistArgs a)
+{
+    const uint64_t nraw = a.counters[kCntCand];
+    const uint64_t n = nraw < a.cand_cap ? nraw : a.cand_cap;
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t idx = (uint64_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += stride) {
+        const uint4 cd = a.cand[idx];
+        const uint32_t slot = cd.y;
+        uint32_t rows = cd.z;
+        const HistSlot h = a.hs[slot];
+        const float* w = a.slot_w + (size_t)slot * kMaxLen * 4;
+        unsigned long long* dst = a.counts + (size_t)slot * a.nbins;
+        while (rows) {
+            const uint32_t r = (uint32_t)__builtin_ctz(rows);
+            rows &= rows - 1u;
+            const uint32_t q = cd.x + 8 * (r >> 2) + (r & 3);
+            const uint64_t bits = load_bits64(a.packed, 2ull * q);
+            const uint64_t mbits = a.lcmask ? load_bits64(a.lcmask, q) : 0ull;
+            float s = 0.f;
+            for (uint32_t p = 0; p < h.len; ++p) {
+                const float4 wp = *(const float4*)(w + 4 * p);
+                const uint32_t b = (uint32_t)(bits >> (2 * p)) & 3u;
+                const float x = b == 0 ? wp.x : b == 1 ? wp.y : b == 2 ? wp.z : wp.w;
+                if (!((mbits >> p) & 1)) s += x;
+            }
+            atomicAdd(&dst[hist_bin_ref(s, h.mn, h.wd, a.nbins)], 1ull);
+        }
+    }
+}
+
+}  // namespace blamm

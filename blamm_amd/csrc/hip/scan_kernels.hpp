@@ -120,7 +120,8 @@ struct Stage {
     uint4* lo;         // bf16: this wave's lo fragments (LDS), [block][chunk][lane]
 };
 
-__device__ __forceinline__ void stage_flush(const FilterArgs& a, Stage& st)
+template <class Args>   // FilterArgs or HistArgs: cand, counters, cand_cap
+__device__ __forceinline__ void stage_flush(const Args& a, Stage& st)
 {
     if (st.n == 0) return;
     const uint32_t lane = threadIdx.x & 63;

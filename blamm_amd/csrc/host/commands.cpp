@@ -80,11 +80,14 @@ void hist_usage()
 {
     std::cout << "Usage: blamm hist [options] motifs.input sequences.input\n"
                  "Goal: compute PWM score histograms\n\n"
-                 " [options]\n  -h\t--help\t\tdisplay help message\n\n"
+                 " [options]\n"
+                 "  -e\t--empirical\tcompute empirical histograms (on the GPU)\n"
+                 "  -h\t--help\t\tdisplay help message\n\n"
                  " [options arg]\n"
                  "  -b\t--numbins\tnumber of bins per histogram [default = 250]\n"
                  "  -t\t--numthreads\tset the number of parallel threads [default = #cores]\n"
-                 "  -l\t--length\tlength of sequence data to analyze [default = 10000000]\n\n"
+                 "  -l\t--length\tlength of sequence data to analyze [default = 10000000]\n"
+                 "  -d\t--device\tHIP device index for -e [default = 0]\n\n"
                  " [file_options]\n"
                  "  -H\t--histdir\toutput directory for the histogram file(s) [default = .]\n\n";
 }
@@ -221,6 +224,66 @@ int dict_main(int argc, char** argv)
     return EXIT_SUCCESS;
 }
 
+// Histogram::generateEmpiricalHist (hist.cpp:140-160) for one group: the
+// scores of every valid window among the group's first maxLength bases,
+// binned on the device (blamm_hip_hist), one histogram per motif written as
+// writeGNUPlotFile does (hist.cpp:255-263).
+static void empirical_histograms(HipContext& hc, int device, const Group& g, const std::vector<Motif>& motifs,
+                          float pseudocount, size_t numBins, size_t maxLength, size_t lmax,
+                          const std::string& histdir)
+{
+    const size_t ncols = motifs.size();
+    if (ncols == 0) return;
+    const size_t ldp = 4 * std::max<size_t>(lmax, 1);
+    std::vector<float> P(ldp * ncols, 0.0f), thr(ncols, 0.0f), mins(ncols), maxs(ncols);
+    std::vector<int> lengths(ncols);
+    for (size_t c = 0; c < ncols; ++c) {
+        pfm_to_pwm(motifs[c], g.counts, pseudocount, &P[c * ldp]);
+        lengths[c] = (int)motifs[c].size();
+        mins[c] = min_score(&P[c * ldp], motifs[c].size());
+        maxs[c] = max_score(&P[c * ldp], motifs[c].size());
+    }
+    if (numBins > (1u << 24)) throw std::runtime_error("number of bins above 2^24");
+    const PackedStream s = pack_fasta(g.files, maxLength);
+    if (!hc.ctx) {
+        int ndev = 0;
+        blamm_hip_device_count(&ndev);
+        if (ndev == 0) throw std::runtime_error("HIP error: no devices found. Aborting...");
+        hip_check(blamm_hip_create(device, &hc.ctx), "device setup");
+    }
+    hip_check(blamm_hip_set_library(hc.ctx, P.data(), (int)ldp, (int)ncols, lengths.data(), thr.data(),
+                                    BLAMM_PREC_BF16X2),
+              "library upload");
+    std::vector<uint64_t> counts(ncols * numBins, 0);
+    // segments of 2^30 bases with a halo of lmax - 1; windows are counted in
+    // the segment where they start
+    const uint64_t seg = 1ull << 30, halo = lmax - 1;
+    for (uint64_t s0 = 0; s0 < s.n; s0 += seg) {
+        const uint64_t payload = std::min(seg, s.n - s0);
+        const uint64_t len = std::min(s.n - s0, payload + halo);
+        std::vector<uint64_t> runs;
+        for (size_t r = 0; r < s.runStart.size(); ++r)
+            if (s.runStart[r] > s0 && s.runStart[r] < s0 + len) runs.push_back(s.runStart[r] - s0);
+        std::vector<uint64_t> codes(blamm_hip_packed_words(len), 0), lower;
+        std::memcpy(codes.data(), s.codes.data() + s0 / 32, ((len + 31) / 32) * 8);
+        if (s.hasLower()) {
+            lower.assign(blamm_hip_mask_words(len), 0);
+            std::memcpy(lower.data(), s.lower.data() + s0 / 64, ((len + 63) / 64) * 8);
+        }
+        hip_check(blamm_hip_hist(hc.ctx, codes.data(), s.hasLower() ? lower.data() : nullptr, len, runs.data(),
+                                 runs.size(), payload, mins.data(), maxs.data(), (uint32_t)numBins, counts.data()),
+                  "histogram");
+    }
+    for (size_t c = 0; c < ncols; ++c) {
+        Histogram h;
+        h.numBins = numBins;
+        h.minScore = mins[c];
+        h.maxScore = maxs[c];
+        h.counts.assign(counts.begin() + (ptrdiff_t)(c * numBins), counts.begin() + (ptrdiff_t)((c + 1) * numBins));
+        h.write(histdir, "hist_" + g.name + "_" + motifs[c].name, motifs[c].name + " (" + g.name + ")");
+    }
+}
+
 int hist_main(int argc, char** argv)
 {
     if (argc < 4) {
@@ -228,6 +291,9 @@ int hist_main(int argc, char** argv)
         return EXIT_FAILURE;
     }
     size_t maxLength = 10000000, numBins = 250;
+    size_t numThreads = std::max(1u, std::thread::hardware_concurrency());
+    bool empirical = false;
+    int device = 0;
     std::string histdir;
     for (int i = 2; i < argc - 2; ++i) {
         const std::string a(argv[i]);
@@ -241,11 +307,13 @@ int hist_main(int argc, char** argv)
             numBins = (size_t)std::atoll(argv[++i]);
             if (numBins < 2) numBins = 2;
         } else if ((a == "-t" || a == "--numthreads") && more) {
-            ++i;
+            numThreads = (size_t)std::max(1, std::atoi(argv[++i]));
         } else if ((a == "-H" || a == "--histdir") && more) {
             histdir = dir_with_slash(argv[++i]);
         } else if (a == "-e" || a == "--empirical") {
-            throw std::runtime_error("empirical histograms (-e) are not available in this build");
+            empirical = true;
+        } else if ((a == "-d" || a == "--device") && more) {
+            device = std::atoi(argv[++i]);
         } else {
             hist_usage();
             return EXIT_FAILURE;
@@ -259,9 +327,30 @@ int hist_main(int argc, char** argv)
     for (const Motif& m : motifs) lmax = std::max(lmax, m.size());
     std::cout << "Loaded " << motifs.size() << " motifs from disk";
     std::cout << "\nMaximum motif size: " << lmax << std::endl;
+    if (empirical) {
+        for (const Motif& m : motifs)
+            if (m.size() < 1 || m.size() > BLAMM_MAX_MOTIF_LEN)
+                throw std::runtime_error("motif " + m.name + " has length " + std::to_string(m.size()) +
+                                         "; this build scores lengths 1.." + std::to_string(BLAMM_MAX_MOTIF_LEN));
+        std::cout << "Using 1 GPU device (split-bf16 MFMA)" << std::endl;
+    } else {
+        std::cout << "Using " << numThreads << " thread(s)" << std::endl;
+    }
+    HipContext hc;
     for (const Group& g : groups) {
         const std::array<float, 4> bg = g.probabilities(st.pseudocount);
-        std::cout << "Generating histograms for species: " << g.name << "\n";
+        {
+            // Species::writeNuclProbabilities (species.cpp:73-82)
+            const auto prec = std::cout.precision(3);
+            std::cout << "Generating histograms for species: " << g.name << " [A: " << 100.0f * bg[0]
+                      << "%, C: " << 100.0f * bg[1] << "%, G: " << 100.0f * bg[2] << "%, T: " << 100.0f * bg[3]
+                      << "%]\n";
+            std::cout.precision(prec);
+        }
+        if (empirical) {
+            empirical_histograms(hc, device, g, motifs, st.pseudocount, numBins, maxLength, lmax, histdir);
+            continue;
+        }
         for (const Motif& m : motifs) {
             std::vector<float> w(4 * m.size());
             pfm_to_pwm(m, g.counts, st.pseudocount, w.data());

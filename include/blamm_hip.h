@@ -297,6 +297,33 @@ int blamm_hip_server_stats(blamm_hip_server *server, blamm_server_stats *stats);
 /* Finishes queued and running requests, then stops the worker. */
 int blamm_hip_server_destroy(blamm_hip_server *server);
 
+/* ------------------------------------------------------------------------
+ * Empirical score histograms (`blamm hist -e`).
+ *
+ * Replace Histogram::histThread (hist.cpp:93-138), whose sgemm over every
+ * offset feeds Histogram::extractObsScore (hist.cpp:70-91): every window of
+ * the stream that satisfies the run rule (hist.cpp:83-86) and starts before
+ * qend adds one count to its column's histogram, in the bin of
+ * ScoreHistogram::addObservation (motif.h:96-102),
+ *   bin = clamp(int((score - min) / width), 0, nbins - 1),
+ *   width = (max - min) / nbins (motif.h:58-66, float arithmetic),
+ * where score is the FP32 sum of the window's weights in position order
+ * (lowercase bases add nothing).  The library must have been uploaded with
+ * BLAMM_PREC_BF16X2 (its thresholds are not used).  min_score/max_score hold
+ * one float per library column; counts (ncols x nbins, column-major by
+ * column: counts[col * nbins + bin]) is added to, so several streams (or
+ * segments, with qend cutting the overlap) can be accumulated.  Synchronous.
+ * ------------------------------------------------------------------------ */
+int blamm_hip_hist(blamm_hip_ctx *ctx, const uint64_t *packed, const uint64_t *lcmask, uint64_t nbases,
+                   const uint64_t *run_starts, uint64_t nruns, uint64_t qend, const float *min_score,
+                   const float *max_score, uint32_t nbins, uint64_t *counts);
+/* As blamm_hip_hist on device-resident buffers (sizes as for
+ * blamm_hip_scan_device). */
+int blamm_hip_hist_device(blamm_hip_ctx *ctx, const uint64_t *d_packed, uint64_t packed_words,
+                          const uint64_t *d_lcmask, uint64_t mask_words, uint64_t nbases,
+                          const uint64_t *d_run_starts, uint64_t nruns, uint64_t qend, const float *min_score,
+                          const float *max_score, uint32_t nbins, uint64_t *counts);
+
 #ifdef __cplusplus
 }
 #endif
