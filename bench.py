@@ -488,6 +488,67 @@ def serve_leg(args):
     return out
 
 
+def hist_leg(args, stream, runs):
+    """Empirical histograms (`blamm hist -e`): the first 10 Mbp of the stream
+    (hist's default -l) against the 1,000 forward motifs, 250 bins.  GPU time
+    per call with the stream resident; the oracle (FP32 position-order scores,
+    16 threads) on the first 1 Mbp for equality and as the CPU figure."""
+    import torch
+    import blamm_amd as ba
+    from blamm_amd import synth
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    n = min(int(stream.size), 10_000_000)
+    sub = stream[:n]
+    rs = runs[runs < n]
+    ms = synth.config1_motifs(seed=args.seed)
+    bg = synth.counts_of(sub)
+    pwms = [ba.pwm(p, bg, 0.25, False) for p in ms.pfms]
+    mn = np.array([ba.min_score(w) for w in pwms], np.float32)
+    mx = np.array([ba.max_score(w) for w in pwms], np.float32)
+    P, lengths, _ = ba.build_pattern_matrix(pwms)
+    nbins = 250
+    sc = ba.HipScanner(0)
+    try:
+        sc.set_library(P, lengths, np.zeros(len(pwms), np.float32), ba.PREC_BF16X2)
+        d_codes, d_runs = to_device(ba.pack_codes(sub), n, rs)
+        torch.cuda.synchronize()
+        for _ in range(max(1, args.warmup)):
+            counts = sc.hist_device(d_codes, None, n, d_runs, len(rs), mn, mx, nbins)
+        reps = max(3, args.steps)
+        kern, exact = [], 0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            counts = sc.hist_device(d_codes, None, n, d_runs, len(rs), mn, mx, nbins)
+            st = sc.stats()
+            kern.append(st.kernel_ms)
+            exact = int(st.band_rescored)
+        ms_call = (time.perf_counter() - t0) / reps * 1e3
+        m = 1_000_000
+        text = np.frombuffer(b"ACGT", np.uint8)[sub[:m]].tobytes()
+        rsm = rs[rs < m]
+        d_c2, d_r2 = to_device(ba.pack_codes(sub[:m]), m, rsm)
+        g_small = sc.hist_device(d_c2, None, m, d_r2, len(rsm), mn, mx, nbins)
+    finally:
+        sc.close()
+    cores = min(16, os.cpu_count() or 1)
+    t1 = time.perf_counter()
+    o_small = oracle.empirical_hist(text, rsm, pwms, mn, mx, nbins, threads=cores)
+    t_cpu = time.perf_counter() - t1
+    windows = int(counts.sum())
+    kms = float(np.median(kern))
+    return {
+        "workload": f"hist -e: first {n} bases x {len(pwms)} forward motifs, {nbins} bins",
+        "ms_per_call": ms_call, "kernel_ms": kms,
+        "value": n * len(pwms) / (ms_call / 1e3) / 1e9, "unit": "Gbp*motifs/s",
+        "value_kernels": n * len(pwms) / (kms / 1e3) / 1e9,
+        "windows": windows, "windows_binned_from_fp32_sum": exact,
+        "identical_to_oracle_first_1Mbp": bool(np.array_equal(g_small, o_small)),
+        "cpu_oracle": {"value": m * len(pwms) / t_cpu / 1e9, "unit": "Gbp*motifs/s", "cores": cores,
+                       "kind": "port", "sample": f"first {m} bases x {len(pwms)} motifs, {t_cpu:.2f} s"},
+    }
+
+
 def cpu_leg(stream, runs, pwms, thr, budget_s: float):
     """The reference's sgemm scan (oracle restatement, OpenBLAS, BLAS threads 1)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -530,8 +591,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--tune-cache", default=os.path.join(os.path.expanduser("~"), ".cache", "blamm_amd", "tune.txt"))
-    ap.add_argument("--sections", default="batched,precision,serve",
-                    help="extra workloads after the headline scan: batched, precision, serve (comma list, or none)")
+    ap.add_argument("--sections", default="batched,precision,serve,hist",
+                    help="extra workloads after the headline scan: batched, precision, serve, hist "
+                         "(comma list, or none)")
     ap.add_argument("--batched-seqs", type=int, default=20_000)
     ap.add_argument("--precision-bases", type=int, default=62_500_000)
     ap.add_argument("--serve-pool", type=int, default=32)
@@ -628,6 +690,10 @@ def main():
         log("serve leg")
         line["serve"] = serve_leg(args)
         log(f"serve: {line['serve']}")
+    if "hist" in sections:
+        log("hist leg")
+        line["hist"] = hist_leg(args, stream, runs)
+        log(f"hist: {line['hist']}")
     if not args.no_cpu:
         log("cpu baseline leg")
         line["cpu_baseline"] = cpu_leg(stream, runs, pwms, thr, args.cpu_budget)

@@ -9,11 +9,14 @@ namespace {
 void hist_slot_params(const blamm_hip_ctx* c, const float* mins, const float* maxs, uint32_t nbins,
                       std::vector<HistSlot>& hs)
 {
-    hs.assign((size_t)c->nslots, HistSlot{0.f, 0.f, 2.f, -1.f, 0.f, 0.f, 0u, 0u});
+    hs.assign((size_t)c->nslots, HistSlot{0.f, 0.f, -1.f, 0.f, 0.f, 0.f, 0u, 0u});
+    std::vector<uint32_t> blk_nch((size_t)c->nslots / kBf16Cols + 1, kMaxChunks);
+    for (const GroupInfo& g : c->groups)
+        for (uint32_t i = 0; i < g.nb; ++i) blk_nch[g.blk0 + i] = g.nch;
     for (int s = 0; s < c->nslots; ++s) {
         const int col = c->slot_col_h[s];
         if (col < 0) continue;
-        HistSlot h{0.f, 0.f, 2.f, -1.f, 0.f, 0.f, 0u, 0u};   // mlo > mhi: every window binned exactly
+        HistSlot h{0.f, 0.f, -1.f, 0.f, 0.f, 0.f, 0u, 0u};   // hm < 0: every window binned exactly
         const int L = c->lib_len[col];
         const float mn = mins[col], mx = maxs[col];
         const float wd = (mx - mn) / (float)nbins;          // motif.h:64
@@ -21,23 +24,37 @@ void hist_slot_params(const blamm_hip_ctx* c, const float* mins, const float* ma
         h.wd = wd;
         h.len = (uint32_t)L;
         const float* w = c->lib_P.data() + (size_t)col * c->ldp;
-        double smax = 0.0;   // bound on |score| of any window
+        double smax = 0.0, ssplit = 0.0, rep = 0.0;   // sum_p max_b |w|, |hi| + |lo|, |w - hi - lo|
         for (int p = 0; p < L; ++p) {
-            double m = 0.0;
-            for (int b = 0; b < 4; ++b) m = std::max(m, (double)std::fabs(w[4 * p + b]));
+            double m = 0.0, ms = 0.0, me = 0.0;
+            for (int b = 0; b < 4; ++b) {
+                const float x = w[4 * p + b];
+                const float hf = bf16_to_float(bf16_rne(x));
+                const float lf = bf16_to_float(bf16_rne(x - hf));
+                m = std::max(m, (double)std::fabs(x));
+                ms = std::max(ms, std::fabs((double)hf) + std::fabs((double)lf));
+                me = std::max(me, std::fabs((double)x - (double)hf - (double)lf));
+            }
             smax += m;
+            ssplit += ms;
+            rep += me;
         }
-        // |MFMA score - FP32 left-to-right score|: half the filter band bounds
-        // the MFMA side against the exact sum, half the sequential band the
-        // FP32 side
-        const double ds = 0.5 * (c->band_h[s] + c->seq_h[s]);
+        // |MFMA score - FP32 left-to-right score|.  MFMA side: the split's
+        // representation error plus the accumulation error.  A window's row
+        // of the one-hot operand has at most 4 nonzero entries per
+        // v_mfma_f32_32x32x16_bf16 (one base per position), so each of the
+        // 2 nch MFMAs adds at most 4 nonzero exact products to its
+        // accumulator: 5 operands, each contributing at most one (possibly
+        // truncating, u = 2^-22) rounding relative to sum_p max(|hi| + |lo|),
+        // i.e. gamma_{10 nch}.  FP32 side: L - 1 round-to-nearest adds.
+        const double ds = rep + gamma_n(10.0 * blk_nch[(size_t)s / kBf16Cols], 0x1p-22) * ssplit +
+                          gamma_n(L, 0x1p-24) * smax;
         if (std::isfinite(mn) && std::isfinite(mx) && wd == 0.f) {
             // (score - min) / 0 is +-inf or NaN for every score; int() of
             // those is INT_MIN, clamped to bin 0
             h.iw = 0.f;
             h.c = 0.5f;
-            h.mlo = 0.f;
-            h.mhi = 1.f;
+            h.hm = 0.5f;
         } else if (std::isfinite(mn) && std::isfinite(wd) && wd > 0.f) {
             const float iw = 1.0f / wd;
             const float cf = -(mn * iw);
@@ -51,14 +68,15 @@ void hist_slot_params(const blamm_hip_ctx* c, const float* mins, const float* ma
             const double m = ds * inv * (1.0 + 0x1p-20) + (S + ds + 2.0 * std::fabs((double)mn)) * inv * 0x1p-22 +
                              (qmax + 1.0) * 0x1p-21 + 0x1p-22;
             if (std::isfinite(iw) && std::isfinite(cf) && qmax < 0x1p30 && m < 0.25) {
-                float lo = (float)m;
-                if ((double)lo < m) lo = std::nextafter(lo, 1.f);
-                float hi = (float)(1.0 - m);
-                if ((double)hi > 1.0 - m) hi = std::nextafter(hi, 0.f);
+                // |fract(x) - 1/2| <= hm implies m <= fract(x) <= 1 - m; the
+                // subtraction is exact for fract(x) >= 1/4 and rounds by at
+                // most 2^-26 below, covered by the extra 2^-25
+                const double hmd = 0.5 - m - 0x1p-25;
+                float hmf = (float)hmd;
+                if ((double)hmf > hmd) hmf = std::nextafter(hmf, 0.f);
                 h.iw = iw;
                 h.c = cf;
-                h.mlo = lo;
-                h.mhi = hi;
+                h.hm = hmf;
             }
         }
         hs[(size_t)s] = h;
@@ -102,14 +120,13 @@ int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
 
     std::vector<HistSlot> hs;
     hist_slot_params(c, min_score, max_score, nbins, hs);
-    // LDS histograms when a block's 32 columns fit; two blocks per item when
-    // both fit
+    // one column block per work item (the kernel keeps one block's
+    // fragments in registers at four waves per SIMD); its histograms live in
+    // LDS when its 32 columns x nbins counters fit
     const bool lds = (uint64_t)nbins * kBf16Cols * 4 <= kHistLdsBytes;
-    const uint32_t hnb = (lds && 2ull * nbins * kBf16Cols * 4 <= kHistLdsBytes) ? 2u : 1u;
     std::vector<uint32_t> groups;
     for (const GroupInfo& g : c->groups)
-        for (uint32_t i = 0; i < g.nb; i += hnb)
-            groups.push_back(group_pack(g.blk0 + i, g.nch, std::min(hnb, g.nb - i)));
+        for (uint32_t i = 0; i < g.nb; ++i) groups.push_back(group_pack(g.blk0 + i, g.nch, 1));
     HIP_TRY(c->hist_slots.reserve(hs.size()));
     HIP_TRY(c->hist_groups.reserve(groups.size()));
     const size_t ncount = (size_t)c->nslots * nbins;
@@ -117,10 +134,8 @@ int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     HIP_TRY(hipMemcpyAsync(c->hist_slots.p, hs.data(), hs.size() * sizeof(HistSlot), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->hist_groups.p, groups.data(), groups.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                            s));
-    if (c->cand.n == 0) HIP_TRY(c->cand.reserve(1u << 20));
-
     const bool mask = d_lcmask != nullptr;
-    const uint32_t dyn = lds ? hnb * kBf16Cols * nbins * 4 : 0u;
+    const uint32_t dyn = lds ? kBf16Cols * nbins * 4 : 0u;
     const void* kp = mask ? (lds ? hist_kernel_ptr<true, true>() : hist_kernel_ptr<true, false>())
                           : (lds ? hist_kernel_ptr<false, true>() : hist_kernel_ptr<false, false>());
     int per_cu = 0;
@@ -154,34 +169,24 @@ int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
         a.nitems = (uint32_t)(nchunks * a.ngroups);
     }
 
-    unsigned long long cnt[kCntSlots];
-    for (int attempt = 0;; ++attempt) {
-        if (attempt == 4) return fail(BLAMM_ERR_DEVICE, "hist: capacity retries exhausted");
-        a.cand = c->cand.p;
-        a.cand_cap = c->cand.n;
-        HIP_TRY(hipMemsetAsync(c->hist_counts.p, 0, ncount * sizeof(unsigned long long), s));
-        hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, c->counters.p);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(c->ev[0], s));
-        if (a.nitems > 0) {
-            if (mask) {
-                if (lds) hipLaunchKernelGGL((hist_filter_kernel<true, true>), dim3(grid), dim3(kThreads), dyn, s, a);
-                else hipLaunchKernelGGL((hist_filter_kernel<true, false>), dim3(grid), dim3(kThreads), dyn, s, a);
-            } else {
-                if (lds) hipLaunchKernelGGL((hist_filter_kernel<false, true>), dim3(grid), dim3(kThreads), dyn, s, a);
-                else hipLaunchKernelGGL((hist_filter_kernel<false, false>), dim3(grid), dim3(kThreads), dyn, s, a);
-            }
-            HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemsetAsync(c->hist_counts.p, 0, ncount * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, c->counters.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[0], s));
+    if (a.nitems > 0) {
+        if (mask) {
+            if (lds) hipLaunchKernelGGL((hist_filter_kernel<true, true>), dim3(grid), dim3(kThreads), dyn, s, a);
+            else hipLaunchKernelGGL((hist_filter_kernel<true, false>), dim3(grid), dim3(kThreads), dyn, s, a);
+        } else {
+            if (lds) hipLaunchKernelGGL((hist_filter_kernel<false, true>), dim3(grid), dim3(kThreads), dyn, s, a);
+            else hipLaunchKernelGGL((hist_filter_kernel<false, false>), dim3(grid), dim3(kThreads), dyn, s, a);
         }
-        HIP_TRY(hipEventRecord(c->ev[1], s));
-        hipLaunchKernelGGL(hist_exact_kernel, dim3((uint32_t)c->num_cu * 8), dim3(kThreads), 0, s, a);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(c->ev[2], s));
-        HIP_TRY(hipStreamSynchronize(s));
-        HIP_TRY(hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
-        if (cnt[kCntCand] <= c->cand.n) break;
-        HIP_TRY(c->cand.reserve((size_t)(cnt[kCntCand] + cnt[kCntCand] / 4 + 1024)));
     }
+    HIP_TRY(hipEventRecord(c->ev[1], s));
+    HIP_TRY(hipStreamSynchronize(s));
+    unsigned long long cnt[kCntSlots];
+    HIP_TRY(hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
     std::vector<unsigned long long> dev(ncount);
     HIP_TRY(hipMemcpy(dev.data(), c->hist_counts.p, ncount * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     for (int sl = 0; sl < c->nslots; ++sl) {
@@ -190,13 +195,12 @@ int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
         for (uint32_t b = 0; b < nbins; ++b)
             counts[(size_t)col * nbins + b] += dev[(size_t)sl * nbins + b];
     }
-    float ms_f = 0.f, ms_all = 0.f;
+    float ms_f = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms_f, c->ev[0], c->ev[1]));
-    HIP_TRY(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]));
     c->stats = blamm_scan_stats{};
-    c->stats.candidates = cnt[kCntCand];
+    c->stats.band_rescored = cnt[kCntBand];   // windows binned from their FP32 sum
     c->stats.filter_ms = ms_f;
-    c->stats.kernel_ms = ms_all;
+    c->stats.kernel_ms = ms_f;
     return BLAMM_OK;
 }
 

@@ -17,8 +17,9 @@
 // fract(x) lies in [m, 1 - m] the FP32 score's coordinate lies in the same
 // unit interval and the bin is trunc(x), clamped.  Counts go to a
 // workgroup-private LDS histogram that is added to HBM once per work item.
-// Windows near a bin edge (about 2m of them) are recorded, and
-// hist_exact_kernel bins their FP32 sum with the reference's arithmetic.
+// Windows near a bin edge (about 2m of them) are staged per wave in LDS and
+// binned exactly by the same wave (FP32 left-to-right sum of the window's
+// weights, the reference's float arithmetic), so no record leaves the chip.
 #pragma once
 
 #include "scan_kernels.hpp"
@@ -27,7 +28,8 @@ namespace blamm {
 
 struct HistSlot {
     float iw, c;       // x = fma(score, iw, c)
-    float mlo, mhi;    // bin decided from x iff mlo <= fract(x) <= mhi
+    float hm;          // bin decided from x iff |fract(x) - 1/2| <= hm
+    float pad0;
     float mn, wd;      // histogram minimum and bin width, as the reference computes them
     uint32_t len;      // motif length; 0 = empty slot
     uint32_t pad;
@@ -41,17 +43,14 @@ struct HistArgs {
     const HistSlot* hs;
     const float* slot_w;           // [slot][kMaxLen][4] FP32 weights
     const uint64_t* run_starts;
-    const uint32_t* groups;        // group_pack(blk0, nch, nb), nb <= kHistNbMax
+    const uint32_t* groups;        // group_pack(blk0, nch, 1): one column block per item
     unsigned long long* counts;    // [slot][nbins]
-    uint4* cand;                   // deferred windows: (first window of the lane, slot, rows, 0)
-    unsigned long long* counters;  // kCntCand: deferred records; kCntQueue: work item head
-    uint64_t cand_cap;
+    unsigned long long* counters;  // kCntBand: windows binned exactly; kCntQueue: work item head
     uint32_t nbases, nruns, qend, nbins;
     uint32_t ntiles, chunk_tiles, ngroups, nitems;
 };
 
-constexpr int kHistNbMax = 2;
-// LDS histogram budget per workgroup (kHistNbMax blocks x 32 columns x 256 bins).
+// LDS histogram budget per workgroup (32 columns x up to 512 bins).
 constexpr uint32_t kHistLdsBytes = 64 * 1024;
 
 // ScoreHistogram::addObservation's bin (motif.h:96-102) of an FP32 score:
@@ -64,6 +63,45 @@ __device__ __forceinline__ uint32_t hist_bin_ref(float s, float mn, float wd, ui
     b = max(b, 0);
     b = min(b, (int)nbins - 1);
     return (uint32_t)b;
+}
+
+// Exact bins of the staged windows (one record per lane: first window of
+// the lane, slot, rows): FP32 left-to-right sum of the window's weights
+// (lowercase bases skipped), the reference's binning, one count each.  The
+// run rule was applied when the record was staged.  Returns the number of
+// windows this lane binned.
+template <bool LDS>
+__device__ __forceinline__ uint32_t hist_exact_stage(const HistArgs& a, Stage& st, uint32_t* hl, uint32_t blk0,
+                                                     uint32_t lane)
+{
+    uint32_t done = 0;
+    for (uint32_t i = lane; i < st.n; i += 64) {
+        const uint4 cd = st.buf[i];
+        const uint32_t slot = cd.y;
+        uint32_t rows = cd.z;
+        const HistSlot h = a.hs[slot];
+        const float* w = a.slot_w + (size_t)slot * kMaxLen * 4;
+        while (rows) {
+            const uint32_t r = (uint32_t)__builtin_ctz(rows);
+            rows &= rows - 1u;
+            const uint32_t q = cd.x + 8 * (r >> 2) + (r & 3);
+            const uint64_t bits = load_bits64(a.packed, 2ull * q);
+            const uint64_t mbits = a.lcmask ? load_bits64(a.lcmask, q) : 0ull;
+            float s = 0.f;
+            for (uint32_t p = 0; p < h.len; ++p) {
+                const float4 wp = *(const float4*)(w + 4 * p);
+                const uint32_t b = (uint32_t)(bits >> (2 * p)) & 3u;
+                const float x = b == 0 ? wp.x : b == 1 ? wp.y : b == 2 ? wp.z : wp.w;
+                if (!((mbits >> p) & 1)) s += x;
+            }
+            const uint32_t bin = hist_bin_ref(s, h.mn, h.wd, a.nbins);
+            if (LDS) atomicAdd(&hl[(slot - blk0 * kBf16Cols) * a.nbins + bin], 1u);
+            else atomicAdd(&a.counts[(size_t)slot * a.nbins + bin], 1ull);
+            ++done;
+        }
+    }
+    st.n = 0;
+    return done;
 }
 
 // Does the window at q of length len end at or before the first run start
@@ -81,15 +119,49 @@ __device__ __forceinline__ bool hist_row_fits(const HistArgs& a, uint32_t q, uin
     return q + len <= r;
 }
 
+// Bins of one finished tile block (16 windows per lane).  A window whose
+// bin the MFMA score decides adds 1 to it; the others add 0 (no branch, one
+// LDS atomic per window) and are returned as a row mask for exact binning.
+// FAST: every window fits its run and starts before qend.
+template <bool FAST, bool LDS>
+__device__ __forceinline__ uint32_t hist_rows(const HistArgs& a, uint32_t* hl, const f32x16& acc, float iw, float cc,
+                                              float hm, uint32_t len, uint32_t slot, uint32_t hoff, uint32_t qlane,
+                                              uint32_t jr, uint32_t rnext)
+{
+    const uint32_t live = len != 0 ? 1u : 0u;
+    const int last = (int)a.nbins - 1;
+    uint32_t undecided = 0, valid = FAST ? (live ? 0xFFFFu : 0u) : 0u;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        uint32_t v = live;
+        if (!FAST) {
+            const uint32_t q = qlane + 8 * (r >> 2) + (r & 3);
+            v = (live && q < a.qend && hist_row_fits(a, q, len, jr, rnext)) ? 1u : 0u;
+            valid |= v << r;
+        }
+        const float x = __builtin_fmaf(acc[r], iw, cc);
+        const bool ok = __builtin_fabsf(__builtin_amdgcn_fractf(x) - 0.5f) <= hm;
+        const uint32_t bin = (uint32_t)min(max((int)x, 0), last);
+        if (LDS) {
+            atomicAdd(&hl[hoff + bin], ok ? v : 0u);
+        } else if (ok && v) {
+            atomicAdd(&a.counts[(size_t)slot * a.nbins + bin], 1ull);
+        }
+        undecided |= (ok ? 0u : 1u) << r;
+    }
+    return undecided & valid;
+}
+
 // Tiles t0, t0 + 4, ... < t1 of column blocks blk0 .. blk0+NB-1 (the wave's
 // share of a work item).  hl: the workgroup's LDS histogram of the item's
 // columns (LDS variant).
 template <int NCH, int NB, bool MASK, bool LDS>
-__device__ __forceinline__ void hist_range(const HistArgs& a, Stage& st, uint32_t* hl, uint32_t blk0, uint32_t t0,
-                                           uint32_t t1, uint32_t lane)
+__device__ __forceinline__ uint32_t hist_range(const HistArgs& a, Stage& st, uint32_t* hl, uint32_t blk0,
+                                               uint32_t t0, uint32_t t1, uint32_t lane)
 {
+    uint32_t exact = 0;
     bf16x8 bh[NB][NCH], bl[NB][NCH];
-    float iw[NB], cc[NB], mlo[NB], mhi[NB];
+    float iw[NB], cc[NB], hm[NB];
     uint32_t len[NB], slot[NB], hoff[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -103,8 +175,7 @@ __device__ __forceinline__ void hist_range(const HistArgs& a, Stage& st, uint32_
         const HistSlot h = a.hs[slot[b]];
         iw[b] = h.iw;
         cc[b] = h.c;
-        mlo[b] = h.mlo;
-        mhi[b] = h.mhi;
+        hm[b] = h.hm;
         len[b] = h.len;
         hoff[b] = LDS ? (uint32_t)(b * kBf16Cols + (lane & 31)) * a.nbins : 0u;
     }
@@ -126,7 +197,6 @@ __device__ __forceinline__ void hist_range(const HistArgs& a, Stage& st, uint32_
         jr = l;
         rnext = l < a.nruns ? (uint32_t)a.run_starts[l] : a.nbases;
     }
-    const uint32_t last = a.nbins - 1;
     for (uint32_t t = t0; t < t1; t += kWavesPerBlock) {
         const uint32_t q0 = t * kBf16Tile;
         const uint64_t bits = funnel64(pk[t], pk[t + 1], sh, sh_inv);
@@ -143,25 +213,14 @@ __device__ __forceinline__ void hist_range(const HistArgs& a, Stage& st, uint32_
         const uint32_t qlane = q0 + 4 * (lane >> 5);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-            uint32_t defer = 0;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t q = qlane + 8 * (r >> 2) + (r & 3);
-                bool v = len[b] != 0;
-                if (!fast) v = v && q < a.qend && hist_row_fits(a, q, len[b], jr, rnext);
-                const float x = __builtin_fmaf(acc[b][r], iw[b], cc[b]);
-                const float fr = __builtin_amdgcn_fractf(x);
-                const bool dec = fr >= mlo[b] && fr <= mhi[b];
-                if (v && dec) {
-                    const uint32_t bin = (uint32_t)min(max((int)x, 0), (int)last);
-                    if (LDS) atomicAdd(&hl[hoff[b] + bin], 1u);
-                    else atomicAdd(&a.counts[(size_t)slot[b] * a.nbins + bin], 1ull);
-                }
-                defer |= (v && !dec ? 1u : 0u) << r;
-            }
+            const uint32_t defer =
+                fast ? hist_rows<true, LDS>(a, hl, acc[b], iw[b], cc[b], hm[b], len[b], slot[b], hoff[b], qlane, jr,
+                                            rnext)
+                     : hist_rows<false, LDS>(a, hl, acc[b], iw[b], cc[b], hm[b], len[b], slot[b], hoff[b], qlane, jr,
+                                             rnext);
             const uint64_t m = __ballot(defer != 0u);
             if (m) {
-                if (st.n > (uint32_t)(kStage - 64)) stage_flush(a, st);
+                if (st.n > (uint32_t)(kStage - 64)) exact += hist_exact_stage<LDS>(a, st, hl, blk0, lane);
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 if (defer) st.buf[st.n + rank] = make_uint4(qlane, slot[b], defer, 0u);
@@ -169,19 +228,24 @@ __device__ __forceinline__ void hist_range(const HistArgs& a, Stage& st, uint32_
             }
         }
     }
+    exact += hist_exact_stage<LDS>(a, st, hl, blk0, lane);
+    return exact;
 }
 
 // Persistent: each workgroup takes work items (chunk of chunk_tiles tiles,
-// column group) from one queue; its four waves share the item's tiles and
-// its LDS histogram.
+// column block) from one queue; its four waves share the item's tiles and
+// its LDS histogram.  Up to 128 VGPRs so that four waves per SIMD hide the
+// latency of the per-window binning (measured: 8.3 ms against 12.3 ms with
+// two blocks per wave at two waves per SIMD, 10 Mbp x 1,000 columns).
 template <bool MASK, bool LDS>
-__global__ __launch_bounds__(kThreads) void hist_filter_kernel(HistArgs a)
+__global__ __launch_bounds__(kThreads, 4) void hist_filter_kernel(HistArgs a)
 {
     extern __shared__ uint32_t hist_lds[];
     __shared__ uint4 stage_lds[kWavesPerBlock][kStage];
     __shared__ uint32_t item_s;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Stage st{stage_lds[wave], 0u, nullptr};
+    uint32_t exact = 0;
     for (;;) {
         if (threadIdx.x == 0) item_s = (uint32_t)atomicAdd(&a.counters[kCntQueue], 1ull);
         __syncthreads();
@@ -196,11 +260,10 @@ __global__ __launch_bounds__(kThreads) void hist_filter_kernel(HistArgs a)
             for (uint32_t i = threadIdx.x; i < used; i += kThreads) hist_lds[i] = 0u;
         __syncthreads();
         switch (nch * 8 + nb) {
-#define BLAMM_H(N, B) case N * 8 + B: hist_range<N, B, MASK, LDS>(a, st, hist_lds, blk0, t0 + wave, t1, lane); break;
+#define BLAMM_H(N, B) \
+    case N * 8 + B: exact += hist_range<N, B, MASK, LDS>(a, st, hist_lds, blk0, t0 + wave, t1, lane); break;
             BLAMM_H(1, 1) BLAMM_H(2, 1) BLAMM_H(3, 1) BLAMM_H(4, 1)
             BLAMM_H(5, 1) BLAMM_H(6, 1) BLAMM_H(7, 1) BLAMM_H(8, 1)
-            BLAMM_H(1, 2) BLAMM_H(2, 2) BLAMM_H(3, 2) BLAMM_H(4, 2)
-            BLAMM_H(5, 2) BLAMM_H(6, 2) BLAMM_H(7, 2) BLAMM_H(8, 2)
 #undef BLAMM_H
             default: break;
         }
@@ -214,40 +277,9 @@ __global__ __launch_bounds__(kThreads) void hist_filter_kernel(HistArgs a)
         }
         // the next item's first barrier orders these reads before its zeroing
     }
-    stage_flush(a, st);
-}
-
-// Exact bins of the deferred windows: FP32 left-to-right sum of the window's
-// weights (lowercase bases skipped) and the reference's float binning.
-// One thread per record; the run rule was applied when the record was made.
-__global__ __launch_bounds__(kThreads) void hist_exact_kernel(HistArgs a)
-{
-    const uint64_t nraw = a.counters[kCntCand];
-    const uint64_t n = nraw < a.cand_cap ? nraw : a.cand_cap;
-    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-    for (uint64_t idx = (uint64_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += stride) {
-        const uint4 cd = a.cand[idx];
-        const uint32_t slot = cd.y;
-        uint32_t rows = cd.z;
-        const HistSlot h = a.hs[slot];
-        const float* w = a.slot_w + (size_t)slot * kMaxLen * 4;
-        unsigned long long* dst = a.counts + (size_t)slot * a.nbins;
-        while (rows) {
-            const uint32_t r = (uint32_t)__builtin_ctz(rows);
-            rows &= rows - 1u;
-            const uint32_t q = cd.x + 8 * (r >> 2) + (r & 3);
-            const uint64_t bits = load_bits64(a.packed, 2ull * q);
-            const uint64_t mbits = a.lcmask ? load_bits64(a.lcmask, q) : 0ull;
-            float s = 0.f;
-            for (uint32_t p = 0; p < h.len; ++p) {
-                const float4 wp = *(const float4*)(w + 4 * p);
-                const uint32_t b = (uint32_t)(bits >> (2 * p)) & 3u;
-                const float x = b == 0 ? wp.x : b == 1 ? wp.y : b == 2 ? wp.z : wp.w;
-                if (!((mbits >> p) & 1)) s += x;
-            }
-            atomicAdd(&dst[hist_bin_ref(s, h.mn, h.wd, a.nbins)], 1ull);
-        }
-    }
+    // windows binned exactly (for statistics): one atomic per wave
+    exact = __reduce_add_sync(~0ull, exact);
+    if (lane == 0 && exact) atomicAdd(&a.counters[kCntBand], (unsigned long long)exact);
 }
 
 }  // namespace blamm

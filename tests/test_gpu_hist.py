@@ -60,7 +60,7 @@ def test_hist_matches_oracle(scanner, orc, seed, lower, lmin, lmax, nbins):
     mn, mx = ranges(pwms)
     got, st = gpu_vs_oracle(scanner, orc, text, runs, pwms, mn, mx, nbins)
     assert int(got.sum()) > 0
-    say(f"  deferred records {st.candidates}")
+    say(f"  windows binned from their FP32 sum: {st.band_rescored}")
 
 
 def test_hist_qend_and_empty(scanner, orc):
@@ -127,7 +127,7 @@ def test_hist_totals_at_scale(scanner, orc):
     want = orc.empirical_hist(text, runs, [pwms[c] for c in sample], mn[sample], mx[sample], 250)
     assert (got[sample] == want).all()
     say(f"  4 Mbp x 300 columns: filter {st.filter_ms:.2f} ms, all {st.kernel_ms:.2f} ms, "
-        f"deferred records {st.candidates}")
+        f"windows binned from their FP32 sum: {st.band_rescored}")
 
 
 def test_hist_needs_split_library(scanner):
