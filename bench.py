@@ -549,6 +549,56 @@ def hist_leg(args, stream, runs):
     }
 
 
+def cli_leg(args):
+    """The `blamm` command line end to end on the same workload written as
+    files (FASTA with 60-column lines, jaspar motifs, one-group manifest):
+    dict, theoretical hist, `scan -rc -pt <p>` into occurrences.txt and into
+    /dev/null (formatting without the file write)."""
+    import shutil
+    import subprocess
+    import tempfile
+    import blamm_amd as ba
+    from blamm_amd import synth
+    d = tempfile.mkdtemp(prefix="blamm_cli_")
+    try:
+        synth.write_fasta(os.path.join(d, "chr.fa"), [synth.config1_record(seed=args.seed, n=args.bases)])
+        synth.config1_motifs(seed=args.seed).write_jaspar(os.path.join(d, "motifs.jaspar"))
+        with open(os.path.join(d, "sequences.mf"), "w") as f:
+            f.write("grp\tchr.fa\n")
+        times, phases = {}, {}
+        pt = f"{args.pvalue:g}"
+        for name, cmd in (("dict", ["dict", "sequences.mf"]),
+                          ("hist", ["hist", "motifs.jaspar", "sequences.mf"]),
+                          ("scan", ["scan", "-rc", "-pt", pt, "motifs.jaspar", "sequences.mf"]),
+                          ("scan_devnull", ["scan", "-rc", "-pt", pt, "-o", "/dev/null", "motifs.jaspar",
+                                            "sequences.mf"])):
+            t0 = time.perf_counter()
+            r = subprocess.run([ba.BLAMM_BIN] + cmd, cwd=d, capture_output=True, text=True,
+                               env={**os.environ, "BLAMM_TIMING": "1"})
+            times[name] = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(f"blamm {' '.join(cmd)} failed: {r.stderr[-500:]}")
+            if name.startswith("scan"):
+                phases[name] = {ln.split("] ", 1)[1].rsplit(" ", 2)[0]: float(ln.rsplit(" ", 2)[1])
+                                for ln in r.stderr.splitlines() if ln.startswith("[blamm timing]")}
+            log(f"cli {name}: {times[name]:.2f} s")
+        out = os.path.join(d, "occurrences.txt")
+        nlines = 0
+        with open(out, "rb") as f:
+            while True:
+                b = f.read(1 << 24)
+                if not b:
+                    break
+                nlines += b.count(b"\n")
+        size = os.path.getsize(out)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"workload": f"blamm dict/hist/scan -rc -pt {pt} on a {args.bases // 1_000_000} Mbp FASTA x "
+                        f"1,000 jaspar motifs", "seconds": times, "scan_phases_ms": phases,
+            "occurrence_lines": nlines,
+            "occurrence_bytes": size, "threads": os.cpu_count()}
+
+
 def cpu_leg(stream, runs, pwms, thr, budget_s: float):
     """The reference's sgemm scan (oracle restatement, OpenBLAS, BLAS threads 1)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -592,8 +642,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--tune-cache", default=os.path.join(os.path.expanduser("~"), ".cache", "blamm_amd", "tune.txt"))
     ap.add_argument("--sections", default="batched,precision,serve,hist",
-                    help="extra workloads after the headline scan: batched, precision, serve, hist "
-                         "(comma list, or none)")
+                    help="extra workloads after the headline scan: batched, precision, serve, hist, cli "
+                         "(comma list, or none; cli writes ~2 GB of occurrences to a temporary directory)")
     ap.add_argument("--batched-seqs", type=int, default=20_000)
     ap.add_argument("--precision-bases", type=int, default=62_500_000)
     ap.add_argument("--serve-pool", type=int, default=32)
@@ -690,6 +740,10 @@ def main():
         log("serve leg")
         line["serve"] = serve_leg(args)
         log(f"serve: {line['serve']}")
+    if "cli" in sections:
+        log("cli leg")
+        line["cli"] = cli_leg(args)
+        log(f"cli: {line['cli']}")
     if "hist" in sections:
         log("hist leg")
         line["hist"] = hist_leg(args, stream, runs)

@@ -5,8 +5,13 @@
 // body differs by design: each manifest group is packed once to 2 bits per
 // base and handed to the MI355X engine (include/blamm_hip.h), which returns the
 // compacted occurrence list; this file only formats it.
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <cerrno>
 #include <charconv>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -142,22 +147,123 @@ struct ColumnInfo {
     char strand;
 };
 
-// One output line per hit: seqName  blamm  motif  pos  pos+L  score  strand  .  .
-// (pwmscan.cpp:88-95); formatted by `threads` workers, written in hit order.
-uint64_t write_hits(std::ostream& os, const std::vector<blamm_hit>& hits, const PackedStream& s,
+// The occurrence file.  Each batch of formatted lines arrives as one string
+// per worker; the workers' strings are written at their final offsets by one
+// thread each (pwrite), so the copy into the page cache runs in parallel.  A
+// destination without offsets (a pipe) is written in order.
+class OutFile {
+public:
+    explicit OutFile(const std::string& path) : path_(path)
+    {
+        fd_ = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+        if (fd_ < 0) throw std::runtime_error("Cannot write to file: " + path);
+        seekable_ = ::lseek(fd_, 0, SEEK_CUR) >= 0;
+    }
+    ~OutFile()
+    {
+        if (fd_ >= 0) ::close(fd_);
+    }
+    OutFile(const OutFile&) = delete;
+    OutFile& operator=(const OutFile&) = delete;
+
+    void write_parts(const std::vector<std::string>& parts)
+    {
+        if (!seekable_) {
+            for (const std::string& p : parts) put(p.data(), p.size(), -1);
+            return;
+        }
+        std::vector<uint64_t> at(parts.size());
+        for (size_t i = 0; i < parts.size(); ++i) {
+            at[i] = off_;
+            off_ += parts[i].size();
+        }
+        std::vector<std::string> err(parts.size());
+        auto one = [&](size_t i) {
+            try {
+                put(parts[i].data(), parts[i].size(), (int64_t)at[i]);
+            } catch (const std::exception& e) {
+                err[i] = e.what();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t i = 1; i < parts.size(); ++i) pool.emplace_back(one, i);
+        if (!parts.empty()) one(0);
+        for (auto& t : pool) t.join();
+        for (const std::string& e : err)
+            if (!e.empty()) throw std::runtime_error(e);
+    }
+
+    void close()
+    {
+        if (fd_ >= 0 && ::close(fd_) != 0) {
+            fd_ = -1;
+            throw std::runtime_error("Cannot write to file: " + path_);
+        }
+        fd_ = -1;
+    }
+
+private:
+    // all of [p, p + n) at offset `at` (or at the current position when at < 0)
+    void put(const char* p, size_t n, int64_t at)
+    {
+        while (n > 0) {
+            const ssize_t w = at >= 0 ? ::pwrite(fd_, p, n, (off_t)at) : ::write(fd_, p, n);
+            if (w < 0) {
+                if (errno == EINTR) continue;
+                throw std::runtime_error("Cannot write to file: " + path_ + " (" + std::strerror(errno) + ")");
+            }
+            p += w;
+            n -= (size_t)w;
+            if (at >= 0) at += w;
+        }
+    }
+
+    std::string path_;
+    int fd_ = -1;
+    bool seekable_ = true;
+    uint64_t off_ = 0;
+};
+
+// Phase timings on stderr when BLAMM_TIMING is set (diagnostics only).
+class PhaseClock {
+public:
+    PhaseClock() : on_(std::getenv("BLAMM_TIMING") != nullptr), t_(std::chrono::steady_clock::now()) {}
+    void tick(const char* what)
+    {
+        const auto now = std::chrono::steady_clock::now();
+        if (on_)
+            std::cerr << "[blamm timing] " << what << " "
+                      << std::chrono::duration<double, std::milli>(now - t_).count() << " ms" << std::endl;
+        t_ = now;
+    }
+
+private:
+    bool on_;
+    std::chrono::steady_clock::time_point t_;
+};
+
+// One output line per hit whose position is below `payload`: seqName  blamm
+// motif  pos  pos+L  score  strand  .  .  (pwmscan.cpp:88-95); formatted by
+// `threads` workers, written in hit order.
+uint64_t write_hits(OutFile& os, const blamm_hit* hits, size_t n, uint64_t payload, const PackedStream& s,
                     uint64_t base, const std::vector<ColumnInfo>& cols, const Group& g, int threads)
 {
-    const size_t n = hits.size();
     if (n == 0) return 0;
     threads = (int)std::max<size_t>(1, std::min<size_t>((size_t)threads, n / 4096 + 1));
     std::vector<std::string> out((size_t)threads);
     std::vector<std::string> err((size_t)threads);
+    std::vector<uint64_t> written((size_t)threads, 0);
     auto work = [&](int t) {
         const size_t b = n * (size_t)t / (size_t)threads, e = n * (size_t)(t + 1) / (size_t)threads;
-        std::string& o = out[(size_t)t];
+        // a local string: the size fields of neighbouring strings in `out`
+        // would share cache lines between workers
+        std::string o;
         o.reserve((e - b) * 64);
+        uint64_t kept = 0;
         for (size_t i = b; i < e; ++i) {
             const blamm_hit& h = hits[i];
+            if (h.pos >= payload) continue;   // starts in the halo: the next segment reports it
+            ++kept;
             uint64_t seq, pos;
             s.locate(base + h.pos, seq, pos);
             if (seq >= g.seqNames.size()) {
@@ -179,6 +285,8 @@ uint64_t write_hits(std::ostream& os, const std::vector<blamm_hit>& hits, const 
             o += c.strand;
             o += "\t.\t.\n";
         }
+        written[(size_t)t] = kept;
+        out[(size_t)t] = std::move(o);
     };
     std::vector<std::thread> pool;
     for (int t = 1; t < threads; ++t) pool.emplace_back(work, t);
@@ -186,8 +294,10 @@ uint64_t write_hits(std::ostream& os, const std::vector<blamm_hit>& hits, const 
     for (auto& th : pool) th.join();
     for (const std::string& e : err)
         if (!e.empty()) throw std::runtime_error(e);
-    for (const std::string& o : out) os.write(o.data(), (std::streamsize)o.size());
-    return n;
+    os.write_parts(out);
+    uint64_t total = 0;
+    for (uint64_t w : written) total += w;
+    return total;
 }
 
 struct HipContext {
@@ -469,9 +579,9 @@ int scan_main(int argc, char** argv)
     const size_t ldp = 4 * std::max<size_t>(lmax, 1);
 
     std::ofstream cutoffs("PWMthresholds.txt");
-    std::ofstream os(outName);
-    if (!os) throw std::runtime_error("Cannot write to file: " + outName);
+    OutFile os(outName);
     uint64_t total = 0;
+    PhaseClock clock;
     for (const Group& g : groups) {
         const std::array<float, 4> bgp = g.probabilities(st.pseudocount);
         {
@@ -504,10 +614,14 @@ int scan_main(int argc, char** argv)
             cutoffs << g.name << "\t" << motifs[c].name << "\t" << min_score(w, motifs[c].size()) << "\t" << thr[c]
                     << "\t" << max_score(w, motifs[c].size()) << std::endl;
         }
+        clock.tick("thresholds");
         const PackedStream s = pack_fasta(g.files, g.totSeqLen);
+        clock.tick("read and pack FASTA");
         open_device();
+        clock.tick("device setup");
         hip_check(blamm_hip_set_library(hc.ctx, P.data(), (int)ldp, (int)ncols, lengths.data(), thr.data(), precision),
                   "library upload");
+        clock.tick("library upload");
         // Segments of the stream (multiples of 64 bases) with a halo of
         // lmax - 1 bases; hits starting in a halo belong to the next segment.
         const uint64_t seg = 1ull << 30;
@@ -551,11 +665,9 @@ int scan_main(int argc, char** argv)
             hip_check(blamm_hip_scan_pinned(hc.ctx, codes.data(), s.hasLower() ? lower.data() : nullptr, len,
                                             runs.data(), runs.size(), &hits, &nh),
                       "scan");
-            std::vector<blamm_hit> kept;
-            kept.reserve(nh);
-            for (uint64_t i = 0; i < nh; ++i)
-                if (hits[i].pos < payload) kept.push_back(hits[i]);
-            total += write_hits(os, kept, s, s0, cols, g, (int)numThreads);
+            clock.tick("scan (upload, kernels, hit read-back)");
+            total += write_hits(os, hits, nh, payload, s, s0, cols, g, (int)numThreads);
+            clock.tick("format and write occurrences");
         }
     }
     os.close();

@@ -125,12 +125,17 @@ def write_manifest(dirpath: str, groups: Sequence[Tuple[str, Sequence[str]]]) ->
 
 # ---- named workloads ---------------------------------------------------------
 
-def config1_sequence(seed: int = 1, n: int = 100_000_000) -> Tuple[np.ndarray, np.ndarray]:
+def config1_record(seed: int = 1, n: int = 100_000_000) -> Record:
     """One 100 Mbp record, GC 0.41, CG at 0.2x, 0.5% N-runs of 10..500."""
     rng = np.random.default_rng(seed)
     codes = bases(n, 0.41, rng, cpg_factor=0.2)
     nm = n_runs(n, 0.005, 10, 500, rng)
-    return filtered([Record("chrSyn", codes, nm)])
+    return Record("chrSyn", codes, nm)
+
+
+def config1_sequence(seed: int = 1, n: int = 100_000_000) -> Tuple[np.ndarray, np.ndarray]:
+    """Filtered stream and run starts of config1_record."""
+    return filtered([config1_record(seed, n)])
 
 
 def config1_motifs(seed: int = 1) -> MotifSet:
