@@ -95,25 +95,30 @@ SIZES = [0, 1, 31, 64, 65, 1000, 4999, 20_000, 33, 50_000, 7, 12_345]
 
 
 @pytest.mark.parametrize("prec", PRECS)
-def test_batch_matches_oracle_per_request(scanner, orc, prec):
+@pytest.mark.parametrize("tuning", [(512, 2, 4), (64, 1, 8)])
+def test_batch_matches_oracle_per_request(scanner, orc, prec, tuning):
     pwms, thr, _ = make_library(61)
     P, lengths, _ = ba.build_pattern_matrix(pwms)
     scanner.set_library(P, lengths, thr, prec)
-    _, err = scanner.bounds()
-    reqs = make_requests(62, pwms, thr, SIZES)
-    b = ba.Batch(scanner, max_bases=1 << 16, max_requests=4)
-    say(f"batch prec={prec} requests={len(reqs)} bases={sum(len(t) for t, _, _ in reqs)}")
-    b.submit([to_request(*r) for r in reqs])
-    total = b.wait()
-    seen = 0
-    for i, (text, runs, t) in enumerate(reqs):
-        ref = oracle_hits(orc, text, runs, pwms, thr if t is None else t)
-        h = b.hits(i)
-        assert_matches(h, ref, err, f"request {i} (n={len(text)})")
-        seen += len(h)
-    assert seen == total
-    assert total > 1000
-    b.close()
+    scanner.set_tuning(*tuning)
+    try:
+        _, err = scanner.bounds()
+        reqs = make_requests(62, pwms, thr, SIZES)
+        b = ba.Batch(scanner, max_bases=1 << 16, max_requests=4)
+        say(f"batch prec={prec} tuning={tuning} requests={len(reqs)} bases={sum(len(t) for t, _, _ in reqs)}")
+        b.submit([to_request(*r) for r in reqs])
+        total = b.wait()
+        seen = 0
+        for i, (text, runs, t) in enumerate(reqs):
+            ref = oracle_hits(orc, text, runs, pwms, thr if t is None else t)
+            h = b.hits(i)
+            assert_matches(h, ref, err, f"request {i} (n={len(text)})")
+            seen += len(h)
+        assert seen == total
+        assert total > 1000
+        b.close()
+    finally:
+        scanner.set_tuning(512, 2, 4)
 
 
 def test_batch_graph_replays_match_eager(scanner, orc):

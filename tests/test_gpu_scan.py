@@ -399,3 +399,38 @@ def test_full_size_plain_gpu_config(scanner, orc):
         for p in range(w.shape[0]):
             s64 += float(w[p, stream[q + p]])
         assert s64 >= float(thr[c]) and abs(s64 - float(h["score"][i])) <= err
+
+
+def test_cli_cluster_buster_permutations(tmp_path):
+    """A cluster-buster motif file (any name not ending in .jaspar,
+    motif.cpp:323-368) with permutation motifs (names ending in "__" + digits,
+    motif.h:256-281): `hist` skips the permutations, `scan -pt` gives each one
+    its base motif's histogram.  Two groups, lowercase, CRLF."""
+    from test_host import _write_synthetic
+    rng = np.random.default_rng(17)
+    names = ["M0", "M0__1", "M0__22", "M1", "M1__3", "X_1", "Y", "Y__", "Z"]
+    lines = []
+    for nm in names:
+        lines.append(">" + nm)
+        for _ in range(int(rng.integers(4, 13))):
+            lines.append(" ".join(str(int(v)) for v in rng.integers(0, 60, size=4)))
+    a, b = tmp_path / "prod", tmp_path / "orc"
+    for d in (a, b):
+        d.mkdir()
+        _write_synthetic(str(d), seed=12, lower=True, crlf=True)
+        (d / "motifs.cb").write_text("\n".join(lines) + "\n")
+    for d, run in ((str(a), run_blamm), (str(b), run_oracle)):
+        run(["dict", "sequences.mf"], d)
+        run(["hist", "motifs.cb", "sequences.mf"], d)
+    hists = sorted(f for f in os.listdir(a) if f.startswith("hist_") and f.endswith(".dat"))
+    assert hists == sorted(f for f in os.listdir(b) if f.startswith("hist_") and f.endswith(".dat"))
+    assert not any("__" in f for f in hists)
+    say("cli cluster-buster scan")
+    for opts in (["-rc", "-pt", "0.001"], ["-rt", "0.8"]):
+        run_blamm(["scan"] + opts + ["motifs.cb", "sequences.mf"], str(a))
+        run_oracle(["scan"] + opts + ["motifs.cb", "sequences.mf"], str(b))
+        n = assert_same_occurrences(str(a / "occurrences.txt"), str(b / "occurrences.txt"))
+        assert n > 0
+        assert open(a / "PWMthresholds.txt").read() == open(b / "PWMthresholds.txt").read()
+        motifs = {k[1] for k in read_occurrences(str(a / "occurrences.txt"))}
+        assert {"M0__1", "M0__22", "Y__"} & motifs
