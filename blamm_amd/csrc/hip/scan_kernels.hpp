@@ -212,13 +212,30 @@ constexpr int kGroup = 4;
 // The one-hot operand of chunk j for the lane's window: bf16 1.0 (0x3F80) in
 // lane b of four per base, built in registers (no memory latency; the VALU
 // work overlaps earlier MFMAs); a lowercase base gets an all-zero row.
-template <bool MASK>
-__device__ __forceinline__ bf16x8 onehot_chunk(uint64_t bits, uint32_t mb, int j)
+// Its source, the lane's bases as shift counts: byte j of p0 / p1 holds 16 x the base
+// at chunk j's first / second position of the lane (bits 4-5 of the byte, the
+// rest zero), so the position's one-hot pair of dwords is 0x3F80 shifted by
+// that byte; v_lshlrev_b64 reads only the low six bits of its count, so a
+// chunk costs one 32-bit shift (none for chunks 0 and 4) and one 64-bit shift
+// per position.
+struct OneHotSrc {
+    uint64_t p0, p1;
+};
+
+__device__ __forceinline__ OneHotSrc onehot_src(uint64_t bits)
 {
-    const uint32_t b0 = (uint32_t)(bits >> (8 * j)) & 3u;
-    const uint32_t b1 = (uint32_t)(bits >> (8 * j + 2)) & 3u;
-    uint64_t o0 = 0x3F80ull << (b0 * 16);
-    uint64_t o1 = 0x3F80ull << (b1 * 16);
+    OneHotSrc s{(bits << 4) & 0x3030303030303030ull, (bits << 2) & 0x3030303030303030ull};
+    // an empty asm (no instruction) keeps the compiler from folding the
+    // masks back into every per-position extraction
+    asm("" : "+v"(s.p0), "+v"(s.p1));
+    return s;
+}
+
+template <bool MASK>
+__device__ __forceinline__ bf16x8 onehot_chunk(const OneHotSrc& src, uint32_t mb, int j)
+{
+    uint64_t o0 = 0x3F80ull << ((uint32_t)(src.p0 >> (8 * j)) & 63u);
+    uint64_t o1 = 0x3F80ull << ((uint32_t)(src.p1 >> (8 * j)) & 63u);
     if (MASK) {
         if ((mb >> (4 * j)) & 1) o0 = 0;
         if ((mb >> (4 * j + 1)) & 1) o1 = 0;
@@ -229,7 +246,7 @@ __device__ __forceinline__ bf16x8 onehot_chunk(uint64_t bits, uint32_t mb, int j
 
 template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const bf16x8 (&bh)[NB][NCH],
-                                               const bf16x8 (&bl)[NB][NCH], uint64_t bits, uint32_t mb)
+                                               const bf16x8 (&bl)[NB][NCH], const OneHotSrc& bits, uint32_t mb)
 {
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
@@ -269,8 +286,8 @@ __device__ __forceinline__ void bf16_epilogue(const FilterArgs& a, Stage& st, co
 // that pass get their lo products and the exact-band test of the plain form.
 // ---------------------------------------------------------------------------
 template <int NCH, int NB, bool MASK>
-__device__ __forceinline__ void bf16_hi_tile(f32x16 (&acc)[NB], const bf16x8 (&bh)[NB][NCH], uint64_t bits,
-                                             uint32_t mb)
+__device__ __forceinline__ void bf16_hi_tile(f32x16 (&acc)[NB], const bf16x8 (&bh)[NB][NCH],
+                                             const OneHotSrc& bits, uint32_t mb)
 {
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
@@ -297,7 +314,7 @@ __device__ __forceinline__ float tile_max(const f32x16& v)
 template <int NCH, int NB, bool MASK, bool LO_LDS>
 __device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st, f32x16 (&acc)[NB],
                                                  const bf16x8 (&bl)[NB][NCH], const float (&th)[NB],
-                                                 const float (&thh)[NB], uint64_t bits, uint32_t mb, uint32_t q0,
+                                                 const float (&thh)[NB], const OneHotSrc& bits, uint32_t mb, uint32_t q0,
                                                  uint32_t blk0, uint32_t lane)
 {
     uint32_t need = 0;
@@ -364,9 +381,9 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
     // hi products of tile t are issued before tile t-1 is finished
     static_assert(NB * NCH <= 2 * kMaxChunks, "lo fragments exceed the wave's LDS slice");
     f32x16 accA[NB], accB[NB];
-    uint64_t bA, bB = 0;
+    OneHotSrc bA, bB{0, 0};
     uint32_t mA = 0, mB = 0, m0 = 0, m1 = 0, m2 = 0;
-    bA = funnel64(pk[t0], pk[t0 + 1], sh, sh_inv);
+    bA = onehot_src(funnel64(pk[t0], pk[t0 + 1], sh, sh_inv));
     if (MASK) mA = funnel32(mk[t0], mk[t0 + 1], off & 31);
     bf16_hi_tile<NCH, NB, MASK>(accA, bh, bA, mA);
     uint64_t w0 = pk[t0 + 1], w1 = pk[t0 + 2], w2 = pk[t0 + 3];
@@ -377,11 +394,11 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         const uint64_t w3 = pk[t + 3], w4 = pk[t + 4];
         uint32_t m3 = 0, m4 = 0;
         if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
-        bB = funnel64(w0, w1, sh, sh_inv);
+        bB = onehot_src(funnel64(w0, w1, sh, sh_inv));
         if (MASK) mB = funnel32(m0, m1, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
         bf16_finish_tile<NCH, NB, MASK, true>(a, st, accA, bl, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
-        bA = funnel64(w1, w2, sh, sh_inv);
+        bA = onehot_src(funnel64(w1, w2, sh, sh_inv));
         if (MASK) mA = funnel32(m1, m2, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accA, bh, bA, mA);
         bf16_finish_tile<NCH, NB, MASK, true>(a, st, accB, bl, th, thh, bB, mB, t * kBf16Tile, blk0, lane);
@@ -389,7 +406,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
     if (t < t1) {                   // one tile left: t
-        bB = funnel64(w0, w1, sh, sh_inv);
+        bB = onehot_src(funnel64(w0, w1, sh, sh_inv));
         if (MASK) mB = funnel32(m0, m1, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
         bf16_finish_tile<NCH, NB, MASK, true>(a, st, accA, bl, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
@@ -410,7 +427,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
     for (uint32_t t = t0; t < t1; ++t) {
         const uint64_t c2 = pk[t + 2];
         const uint32_t cm2 = MASK ? mk[t + 2] : 0u;
-        const uint64_t bits = funnel64(c0, c1, sh, sh_inv);
+        const OneHotSrc bits = onehot_src(funnel64(c0, c1, sh, sh_inv));
         const uint32_t mb = MASK ? funnel32(cm0, cm1, off & 31) : 0u;
         bf16_hi_tile<NCH, NB, MASK>(acc, bh, bits, mb);
         mfma_result_guard();
@@ -430,7 +447,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         const uint32_t q0m = MASK ? mk[t0] : 0u;
         uint64_t p1 = pk[t0 + 1];
         if (MASK) m0 = mk[t0 + 1];
-        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, funnel64(p0, p1, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, onehot_src(funnel64(p0, p1, sh, sh_inv)),
                                       MASK ? funnel32(q0m, m0, off & 31) : 0u);
     }
     uint64_t w0 = pk[t0 + 1], w1 = pk[t0 + 2], w2 = pk[t0 + 3];
@@ -441,17 +458,17 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         const uint64_t w3 = pk[t + 3], w4 = pk[t + 4];
         uint32_t m3 = 0, m4 = 0;
         if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
-        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, funnel64(w0, w1, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, onehot_src(funnel64(w0, w1, sh, sh_inv)),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
         bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
-        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, funnel64(w1, w2, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, onehot_src(funnel64(w1, w2, sh, sh_inv)),
                                       MASK ? funnel32(m1, m2, off & 31) : 0u);
         bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
         w0 = w2; w1 = w3; w2 = w4;
         if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
     if (t < t1) {                   // one tile left: t
-        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, funnel64(w0, w1, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, onehot_src(funnel64(w0, w1, sh, sh_inv)),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
         bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
         pendingA = false;
