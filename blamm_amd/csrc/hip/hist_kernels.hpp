@@ -199,7 +199,7 @@ __device__ __forceinline__ uint32_t hist_range(const HistArgs& a, Stage& st, uin
     }
     for (uint32_t t = t0; t < t1; t += kWavesPerBlock) {
         const uint32_t q0 = t * kBf16Tile;
-        const OneHotSrc bits = onehot_src(funnel64(pk[t], pk[t + 1], sh, sh_inv));
+        const OneHotSrc bits = onehot_src<NCH>(pk[t], pk[t + 1], sh, sh_inv);
         const uint32_t mb = MASK ? funnel32(mk[t], mk[t + 1], off & 31) : 0u;
         f32x16 acc[NB];
         bf16_mfma_tile<NCH, NB, MASK>(acc, bh, bl, bits, mb);
