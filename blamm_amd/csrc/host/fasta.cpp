@@ -10,10 +10,20 @@
 //  - sequence data before the first record is an error;
 //  - the stream stops at maxLen bases (the dict's TOT_SEQ_LENGTH).
 // Instead of strings and per-block marker maps, the stream is packed to 2 bits
-// per base as it is read, with a run table (start, record, position).
+// per base, with a run table (start, record, position), by all host threads:
+// see pack_fasta.
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cerrno>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
+#include <thread>
 
 #include "host.hpp"
 
@@ -99,58 +109,295 @@ uint64_t PackedStream::nextBreak(uint64_t q) const
     return lo < runStart.size() ? runStart[lo] : n;
 }
 
-PackedStream pack_fasta(const std::vector<std::string>& files, uint64_t maxLen)
+// Smallest chunk a file is cut into for the parallel passes (tests build the
+// reader with tiny chunks to put chunk boundaries everywhere).
+#ifndef BLAMM_FASTA_MIN_CHUNK
+#define BLAMM_FASTA_MIN_CHUNK (1 << 20)
+#endif
+
+namespace {
+
+// One file, read whole (parallel pread) and cut into line-aligned chunks.
+struct FileData {
+    std::string bytes;
+    std::vector<std::pair<size_t, size_t>> chunks;   // [begin, end) at line starts
+};
+
+FileData read_file(const std::string& fn, unsigned threads)
 {
-    Packer pk(maxLen);
-    uint64_t nseq = 0, recPos = 0;
-    std::unique_ptr<char[]> buf(new char[1 << 22]);
-    for (const std::string& fn : files) {
-        if (pk.full) break;
-        FILE* f = std::fopen(fn.c_str(), "rb");
-        if (!f) throw std::runtime_error("Could not open file: " + fn);
-        // line state: 0 = at line start, 1 = in header, 2 = in sequence line
-        int state = 0, nameState = 0;
-        size_t got;
-        while (!pk.full && (got = std::fread(buf.get(), 1, 1 << 22, f)) > 0) {
-            for (size_t i = 0; i < got && !pk.full; ++i) {
-                const uint8_t ch = (uint8_t)buf[i];
-                if (ch == '\n') {
-                    state = 0;
-                    continue;
+    FileData d;
+    const int fd = ::open(fn.c_str(), O_RDONLY);
+    if (fd < 0) throw std::runtime_error("Could not open file: " + fn);
+    struct stat st;
+    if (::fstat(fd, &st) != 0) {
+        ::close(fd);
+        throw std::runtime_error("Could not open file: " + fn);
+    }
+    const size_t size = (size_t)st.st_size;
+    d.bytes.resize(size);
+    const size_t piece = std::max<size_t>(size / threads + 1, 1 << 20);
+    std::vector<std::thread> pool;
+    std::vector<int> bad((size + piece - 1) / piece, 0);
+    for (size_t k = 0; k * piece < size; ++k)
+        pool.emplace_back([&, k]() {
+            size_t off = k * piece;
+            const size_t end = std::min(size, off + piece);
+            while (off < end) {
+                const ssize_t r = ::pread(fd, &d.bytes[off], end - off, (off_t)off);
+                if (r <= 0) {
+                    if (r < 0 && errno == EINTR) continue;
+                    bad[k] = 1;
+                    return;
                 }
-                if (state == 0) {
-                    if (ch == '>') {
-                        state = 1;
-                        nameState = 0;
-                        ++nseq;
-                        recPos = 0;
-                        pk.s.names.emplace_back();
-                        continue;
+                off += (size_t)r;
+            }
+        });
+    for (auto& t : pool) t.join();
+    ::close(fd);
+    for (int b : bad)
+        if (b) throw std::runtime_error("Could not read file: " + fn);
+    // chunk boundaries just after a '\n' (or the file ends)
+    const size_t want = std::max<size_t>(size / (4 * threads) + 1, BLAMM_FASTA_MIN_CHUNK);
+    size_t b = 0;
+    while (b < size) {
+        size_t e = std::min(size, b + want);
+        while (e < size && d.bytes[e - 1] != '\n') ++e;
+        d.chunks.emplace_back(b, e);
+        b = e;
+    }
+    return d;
+}
+
+// What a chunk contributes, independent of what came before it.
+struct ChunkSummary {
+    uint64_t nrec = 0;            // '>' lines
+    bool seqBeforeRec = false;    // a sequence line before its first '>' line
+    uint64_t tailPos = 0;         // record positions advanced after its last '>' (all, if none)
+    uint64_t nvalid = 0;          // ACGTacgt bytes
+    bool lower = false;
+    // the last valid base: record ('>' lines before it in the chunk) and
+    // position (relative to the chunk's starting position when rec == 0)
+    bool hasValid = false;
+    uint64_t lastRec = 0, lastPos = 0;
+};
+
+ChunkSummary summarize(const char* p, size_t n)
+{
+    ChunkSummary c;
+    uint64_t pos = 0;        // position in the current record (relative if no '>' yet)
+    size_t i = 0;
+    while (i < n) {
+        // one line [i, e)
+        const char* nl = (const char*)std::memchr(p + i, '\n', n - i);
+        const size_t e = nl ? (size_t)(nl - p) : n;
+        if (e > i) {
+            if (p[i] == '>') {
+                ++c.nrec;
+                pos = 0;
+            } else {
+                if (c.nrec == 0) c.seqBeforeRec = true;
+                for (size_t k = i; k < e; ++k) {
+                    const uint8_t t = kTable.t[(uint8_t)p[k]];
+                    if (t & 4) {
+                        ++c.nvalid;
+                        c.lower |= (t & 8) != 0;
+                        c.hasValid = true;
+                        c.lastRec = c.nrec;
+                        c.lastPos = pos;
                     }
-                    if (nseq == 0) {
-                        std::fclose(f);
-                        throw std::runtime_error("Input file does not appear to be in fasta format\n");
-                    }
-                    state = 2;
+                    ++pos;
                 }
-                if (state == 1) {
-                    // record name: first whitespace-delimited token after '>'
-                    const bool ws = ch == ' ' || ch == '\t' || ch == '\v' || ch == '\f' || ch == '\r';
-                    if (nameState == 0 && !ws) nameState = 1;
-                    else if (nameState == 1 && ws) nameState = 2;
-                    if (nameState == 1) pk.s.names.back().push_back((char)ch);
-                    continue;
-                }
-                const uint8_t t = kTable.t[ch];
-                if (t & 4) pk.append(t, nseq - 1, recPos);
-                ++recPos;
             }
         }
-        std::fclose(f);
+        i = e + 1;
     }
-    pk.finish();
-    pk.s.nseq = nseq;
-    return std::move(pk.s);
+    c.tailPos = pos;
+    return c;
+}
+
+// State at a chunk start.
+struct ChunkStart {
+    uint64_t nseq = 0, recPos = 0, n0 = 0;
+    bool havePrev = false;
+    uint64_t prevSeq = 0, prevPos = 0;
+};
+
+// Output of one chunk's packing pass (runs and names in order).
+struct ChunkOut {
+    std::vector<uint64_t> runStart, runSeq, runPos;
+    std::vector<std::string> names;
+    uint64_t nrec = 0;      // '>' lines before the chunk's stop point
+    uint64_t nbases = 0;    // bases appended
+};
+
+inline void or_word(uint64_t* w, uint64_t v)
+{
+    if (v) __atomic_fetch_or(w, v, __ATOMIC_RELAXED);
+}
+
+// Pack the chunk's valid bases at stream offsets st.n0 ...; stop after the
+// base with stream index limit - 1.
+ChunkOut pack_chunk(const char* p, size_t n, const ChunkStart& st, uint64_t limit, uint64_t* codes,
+                    uint64_t* lower)
+{
+    ChunkOut o;
+    uint64_t nseq = st.nseq, recPos = st.recPos, q = st.n0;
+    bool havePrev = st.havePrev;
+    uint64_t prevSeq = st.prevSeq, prevPos = st.prevPos;
+    uint64_t word = 0, lword = 0;
+    uint64_t wi = q >> 5, li = q >> 6;
+    size_t i = 0;
+    bool stop = q >= limit;
+    while (i < n && !stop) {
+        const char* nl = (const char*)std::memchr(p + i, '\n', n - i);
+        const size_t e = nl ? (size_t)(nl - p) : n;
+        if (e > i) {
+            if (p[i] == '>') {
+                ++nseq;
+                ++o.nrec;
+                recPos = 0;
+                // record name: first whitespace-delimited token after '>'
+                size_t k = i + 1;
+                auto ws = [](char ch) { return ch == ' ' || ch == '\t' || ch == '\v' || ch == '\f' || ch == '\r'; };
+                while (k < e && ws(p[k])) ++k;
+                size_t m = k;
+                while (m < e && !ws(p[m])) ++m;
+                o.names.emplace_back(p + k, m - k);
+            } else {
+                for (size_t k = i; k < e; ++k) {
+                    const uint8_t t = kTable.t[(uint8_t)p[k]];
+                    if (t & 4) {
+                        const uint64_t seq = nseq - 1;
+                        if (!(havePrev && prevSeq == seq && prevPos + 1 == recPos)) {
+                            o.runStart.push_back(q);
+                            o.runSeq.push_back(seq);
+                            o.runPos.push_back(recPos);
+                        }
+                        havePrev = true;
+                        prevSeq = seq;
+                        prevPos = recPos;
+                        if ((q >> 5) != wi) {
+                            or_word(&codes[wi], word);
+                            word = 0;
+                            wi = q >> 5;
+                        }
+                        word |= (uint64_t)(t & 3) << (2 * (q & 31));
+                        if (lower) {
+                            if ((q >> 6) != li) {
+                                or_word(&lower[li], lword);
+                                lword = 0;
+                                li = q >> 6;
+                            }
+                            if (t & 8) lword |= 1ull << (q & 63);
+                        }
+                        ++q;
+                        ++o.nbases;
+                        if (q >= limit) {
+                            stop = true;
+                            break;
+                        }
+                    }
+                    ++recPos;
+                }
+            }
+        }
+        i = e + 1;
+    }
+    if (o.nbases) {
+        or_word(&codes[wi], word);
+        if (lower) or_word(&lower[li], lword);
+    }
+    return o;
+}
+
+}  // namespace
+
+// Files are read whole and packed in parallel: chunk summaries (records,
+// positions, valid bases), a prefix over them, then every chunk packs its bases
+// at their final stream offsets.  The result is the serial reader's, including
+// where it stops: at maxLen bases, before any later record or file.
+PackedStream pack_fasta(const std::vector<std::string>& files, uint64_t maxLen)
+{
+    const unsigned threads = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    PackedStream s;
+    ChunkStart cur;           // reader state at the start of the next chunk
+    std::vector<uint64_t> codes, lower;
+    bool anyLower = false;
+    auto parallel = [&](size_t count, const std::function<void(size_t)>& fn) {
+        std::vector<std::thread> pool;
+        std::atomic<size_t> next{0};
+        for (unsigned t = 0; t < std::min<size_t>(threads, count); ++t)
+            pool.emplace_back([&]() {
+                for (size_t k; (k = next.fetch_add(1)) < count;) fn(k);
+            });
+        for (auto& t : pool) t.join();
+    };
+    for (const std::string& fn : files) {
+        if (cur.n0 >= maxLen) break;   // the serial reader stops before opening the file
+        const FileData d = read_file(fn, threads);
+        const size_t nc = d.chunks.size();
+        std::vector<ChunkSummary> sum(nc);
+        parallel(nc, [&](size_t k) {
+            sum[k] = summarize(d.bytes.data() + d.chunks[k].first, d.chunks[k].second - d.chunks[k].first);
+        });
+        // each chunk's starting state; `used` chunks are read, the last of
+        // them only up to the maxLen-th base when the stream fills inside it
+        std::vector<ChunkStart> start(nc);
+        size_t used = 0;
+        bool filled = false;
+        ChunkStart c = cur;
+        for (size_t k = 0; k < nc; ++k) {
+            start[k] = c;
+            used = k + 1;
+            if (sum[k].seqBeforeRec && c.nseq == 0)
+                throw std::runtime_error("Input file does not appear to be in fasta format\n");
+            const uint64_t take = std::min<uint64_t>(sum[k].nvalid, maxLen - c.n0);
+            if (sum[k].lower && take > 0) anyLower = true;
+            if (c.n0 + take >= maxLen) {
+                c.n0 += take;
+                filled = true;
+                break;
+            }
+            if (sum[k].hasValid) {
+                c.havePrev = true;
+                c.prevSeq = c.nseq + sum[k].lastRec - 1;
+                c.prevPos = sum[k].lastRec ? sum[k].lastPos : c.recPos + sum[k].lastPos;
+            }
+            c.recPos = sum[k].nrec ? sum[k].tailPos : c.recPos + sum[k].tailPos;
+            c.nseq += sum[k].nrec;
+            c.n0 += take;
+        }
+        codes.resize((size_t)((c.n0 + 31) / 32), 0);
+        if (anyLower) lower.resize((size_t)((c.n0 + 63) / 64), 0);
+        std::vector<ChunkOut> out(used);
+        parallel(used, [&](size_t k) {
+            out[k] = pack_chunk(d.bytes.data() + d.chunks[k].first, d.chunks[k].second - d.chunks[k].first,
+                                start[k], maxLen, codes.data(), anyLower ? lower.data() : nullptr);
+        });
+        for (size_t k = 0; k < used; ++k) {
+            s.runStart.insert(s.runStart.end(), out[k].runStart.begin(), out[k].runStart.end());
+            s.runSeq.insert(s.runSeq.end(), out[k].runSeq.begin(), out[k].runSeq.end());
+            s.runPos.insert(s.runPos.end(), out[k].runPos.begin(), out[k].runPos.end());
+            for (std::string& nm : out[k].names) s.names.push_back(std::move(nm));
+        }
+        if (filled) {
+            // records are counted up to the stop point only
+            cur = start[used - 1];
+            cur.nseq += out[used - 1].nrec;
+            cur.n0 += out[used - 1].nbases;
+            break;
+        }
+        cur = c;
+    }
+    s.n = cur.n0;
+    s.nseq = cur.nseq;
+    s.codes = std::move(codes);
+    s.codes.resize((size_t)((s.n + 31) / 32), 0);
+    if (anyLower) {
+        s.lower = std::move(lower);
+        s.lower.resize((size_t)((s.n + 63) / 64), 0);
+    }
+    return s;
 }
 
 PackedStream pack_bases(const char* bases, uint64_t n)
