@@ -667,7 +667,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              f"(the scan shards by stream range with no collective; see the design notes)", file=sys.stderr)
+              f"(a multi-GPU scan would shard the stream by range with an Lmax-1 halo and no collective)", file=sys.stderr)
         sys.exit(2)
     import blamm_amd as ba
     if args.tune_cache:

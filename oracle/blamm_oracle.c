@@ -37,7 +37,7 @@
  * HIST_FP64); tests/test_oracle_mutations.py
  * shows each is caught by a pinning check.
  * The reference itself is not built here: it needs the CMake-generated
- * config.h and an external BLAS (see the design notes), so no run of the
+ * config.h and an external BLAS, so no run of the
  * reference binary pins any of the above.
  */
 #define _GNU_SOURCE
