@@ -843,14 +843,18 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
                 bits = load_bits64(a.packed, 2ull * q);
                 if (a.lcmask) mbits = load_bits64(a.lcmask, q);
             }
-            // left-to-right FP32 sum of the window
+            // left-to-right FP32 sum of the window: each lane loads only the
+            // weight of its base, four positions at a time so the loads
+            // overlap (rows up to kMaxLen exist for every slot)
             float s = 0.f;
             const int Lw = act ? L : 0;
-            for (int p = 0; p < Lw; ++p) {
-                const float4 wp = *(const float4*)(w + 4 * p);
-                const uint32_t b = (uint32_t)(bits >> (2 * p)) & 3u;
-                const float x = b == 0 ? wp.x : b == 1 ? wp.y : b == 2 ? wp.z : wp.w;
-                if (!((mbits >> p) & 1)) s += x;
+            for (int p0 = 0; p0 < Lw; p0 += 4) {
+                float x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) x[u] = w[4 * (p0 + u) + (uint32_t)((bits >> (2 * (p0 + u))) & 3u)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (p0 + u < Lw && !((mbits >> (p0 + u)) & 1)) s += x[u];
             }
             bool hit = false;
             float score = s;
