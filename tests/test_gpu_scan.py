@@ -434,3 +434,29 @@ def test_cli_cluster_buster_permutations(tmp_path):
         assert open(a / "PWMthresholds.txt").read() == open(b / "PWMthresholds.txt").read()
         motifs = {k[1] for k in read_occurrences(str(a / "occurrences.txt"))}
         assert {"M0__1", "M0__22", "Y__"} & motifs
+
+
+def test_cli_reference_cpu_config(tmp_path):
+    """The reference's CPU-runnable configuration end to end: 64 motifs
+    (L 6-20) and 4 records of 250 kbp with 1% N-runs, `dict`, theoretical
+    `hist`, `scan -rc -pt 1e-4`, against the oracle's command line."""
+    ms, recs = synth.config0(seed=0)
+    a, b = tmp_path / "prod", tmp_path / "orc"
+    for d in (a, b):
+        d.mkdir()
+        synth.write_fasta(str(d / "c0.fa"), recs)
+        ms.write_jaspar(str(d / "motifs.jaspar"))
+        (d / "sequences.mf").write_text("c0\tc0.fa\n")
+    for d, run in ((str(a), run_blamm), (str(b), run_oracle)):
+        run(["dict", "sequences.mf"], d)
+        run(["hist", "motifs.jaspar", "sequences.mf"], d)
+    say("cli reference CPU config scan")
+    run_blamm(["scan", "-rc", "-pt", "1e-4", "motifs.jaspar", "sequences.mf"], str(a))
+    run_oracle(["scan", "-rc", "-pt", "1e-4", "motifs.jaspar", "sequences.mf"], str(b))
+    n = assert_same_occurrences(str(a / "occurrences.txt"), str(b / "occurrences.txt"))
+    assert n > 1000
+    # lines follow the length sort of the motifs, whose tie order is not
+    # fixed (std::sort is unstable, motif.cpp:436): compare them as sets
+    ta = sorted(open(a / "PWMthresholds.txt").read().splitlines())
+    tb = sorted(open(b / "PWMthresholds.txt").read().splitlines())
+    assert ta == tb and len(ta) > 50
