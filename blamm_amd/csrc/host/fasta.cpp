@@ -114,59 +114,108 @@ uint64_t PackedStream::nextBreak(uint64_t q) const
 #ifndef BLAMM_FASTA_MIN_CHUNK
 #define BLAMM_FASTA_MIN_CHUNK (1 << 20)
 #endif
+#ifndef BLAMM_FASTA_SEGMENT
+#define BLAMM_FASTA_SEGMENT (256u << 20)
+#endif
 
 namespace {
 
-// One file, read whole (parallel pread) and cut into line-aligned chunks.
+// A segment of a file (whole lines) cut into line-aligned chunks.
 struct FileData {
     std::string bytes;
     std::vector<std::pair<size_t, size_t>> chunks;   // [begin, end) at line starts
 };
 
-FileData read_file(const std::string& fn, unsigned threads)
-{
-    FileData d;
-    const int fd = ::open(fn.c_str(), O_RDONLY);
-    if (fd < 0) throw std::runtime_error("Could not open file: " + fn);
-    struct stat st;
-    if (::fstat(fd, &st) != 0) {
-        ::close(fd);
-        throw std::runtime_error("Could not open file: " + fn);
+// A file read in line-aligned segments of about BLAMM_FASTA_SEGMENT bytes
+// (each read with parallel pread), so reading can stop once the stream is full
+// and memory stays bounded on large genomes.
+class SegmentReader {
+public:
+    SegmentReader(const std::string& fn, unsigned threads) : fn_(fn), threads_(threads)
+    {
+        fd_ = ::open(fn.c_str(), O_RDONLY);
+        if (fd_ < 0) throw std::runtime_error("Could not open file: " + fn);
+        struct stat st;
+        if (::fstat(fd_, &st) != 0) throw std::runtime_error("Could not open file: " + fn);
+        size_ = (size_t)st.st_size;
     }
-    const size_t size = (size_t)st.st_size;
-    d.bytes.resize(size);
-    const size_t piece = std::max<size_t>(size / threads + 1, 1 << 20);
-    std::vector<std::thread> pool;
-    std::vector<int> bad((size + piece - 1) / piece, 0);
-    for (size_t k = 0; k * piece < size; ++k)
-        pool.emplace_back([&, k]() {
-            size_t off = k * piece;
-            const size_t end = std::min(size, off + piece);
-            while (off < end) {
-                const ssize_t r = ::pread(fd, &d.bytes[off], end - off, (off_t)off);
-                if (r <= 0) {
-                    if (r < 0 && errno == EINTR) continue;
-                    bad[k] = 1;
-                    return;
-                }
-                off += (size_t)r;
+    ~SegmentReader()
+    {
+        if (fd_ >= 0) ::close(fd_);
+    }
+    SegmentReader(const SegmentReader&) = delete;
+    SegmentReader& operator=(const SegmentReader&) = delete;
+
+    // The next segment: whole lines (the file's last line may lack its '\n');
+    // false when the file is exhausted.
+    bool next(FileData& d)
+    {
+        d.chunks.clear();
+        d.bytes.swap(carry_);
+        carry_.clear();
+        if (d.bytes.empty() && off_ >= size_) return false;
+        size_t cut = std::string::npos;
+        while (off_ < size_) {
+            const size_t want = std::min<size_t>(BLAMM_FASTA_SEGMENT, size_ - off_);
+            const size_t base = d.bytes.size();
+            d.bytes.resize(base + want);
+            read_at(&d.bytes[base], want, off_);
+            off_ += want;
+            const size_t nl = d.bytes.rfind('\n');
+            if (nl != std::string::npos && nl >= base) {
+                cut = nl + 1;
+                break;
             }
-        });
-    for (auto& t : pool) t.join();
-    ::close(fd);
-    for (int b : bad)
-        if (b) throw std::runtime_error("Could not read file: " + fn);
-    // chunk boundaries just after a '\n' (or the file ends)
-    const size_t want = std::max<size_t>(size / (4 * threads) + 1, BLAMM_FASTA_MIN_CHUNK);
-    size_t b = 0;
-    while (b < size) {
-        size_t e = std::min(size, b + want);
-        while (e < size && d.bytes[e - 1] != '\n') ++e;
-        d.chunks.emplace_back(b, e);
-        b = e;
+            // no line end in the new bytes: a line longer than a segment
+        }
+        if (off_ < size_ && cut != std::string::npos) {
+            carry_.assign(d.bytes, cut, std::string::npos);
+            d.bytes.resize(cut);
+        }
+        const size_t size = d.bytes.size();
+        const size_t chunk = std::max<size_t>(size / (4 * threads_) + 1, BLAMM_FASTA_MIN_CHUNK);
+        size_t b = 0;
+        while (b < size) {
+            size_t e = std::min(size, b + chunk);
+            while (e < size && d.bytes[e - 1] != '\n') ++e;
+            d.chunks.emplace_back(b, e);
+            b = e;
+        }
+        return true;
     }
-    return d;
-}
+
+private:
+    void read_at(char* dst, size_t n, size_t at)
+    {
+        const size_t piece = std::max<size_t>(n / threads_ + 1, 1 << 20);
+        const size_t np = (n + piece - 1) / piece;
+        std::vector<int> bad(np, 0);
+        std::vector<std::thread> pool;
+        for (size_t k = 0; k < np; ++k)
+            pool.emplace_back([&, k]() {
+                size_t o = k * piece;
+                const size_t end = std::min(n, o + piece);
+                while (o < end) {
+                    const ssize_t r = ::pread(fd_, dst + o, end - o, (off_t)(at + o));
+                    if (r <= 0) {
+                        if (r < 0 && errno == EINTR) continue;
+                        bad[k] = 1;
+                        return;
+                    }
+                    o += (size_t)r;
+                }
+            });
+        for (auto& t : pool) t.join();
+        for (int x : bad)
+            if (x) throw std::runtime_error("Could not read file: " + fn_);
+    }
+
+    std::string fn_;
+    unsigned threads_;
+    int fd_ = -1;
+    size_t size_ = 0, off_ = 0;
+    std::string carry_;
+};
 
 // What a chunk contributes, independent of what came before it.
 struct ChunkSummary {
@@ -332,62 +381,67 @@ PackedStream pack_fasta(const std::vector<std::string>& files, uint64_t maxLen)
             });
         for (auto& t : pool) t.join();
     };
+    bool done = false;
     for (const std::string& fn : files) {
-        if (cur.n0 >= maxLen) break;   // the serial reader stops before opening the file
-        const FileData d = read_file(fn, threads);
-        const size_t nc = d.chunks.size();
-        std::vector<ChunkSummary> sum(nc);
-        parallel(nc, [&](size_t k) {
-            sum[k] = summarize(d.bytes.data() + d.chunks[k].first, d.chunks[k].second - d.chunks[k].first);
-        });
-        // each chunk's starting state; `used` chunks are read, the last of
-        // them only up to the maxLen-th base when the stream fills inside it
-        std::vector<ChunkStart> start(nc);
-        size_t used = 0;
-        bool filled = false;
-        ChunkStart c = cur;
-        for (size_t k = 0; k < nc; ++k) {
-            start[k] = c;
-            used = k + 1;
-            if (sum[k].seqBeforeRec && c.nseq == 0)
-                throw std::runtime_error("Input file does not appear to be in fasta format\n");
-            const uint64_t take = std::min<uint64_t>(sum[k].nvalid, maxLen - c.n0);
-            if (sum[k].lower && take > 0) anyLower = true;
-            if (c.n0 + take >= maxLen) {
+        if (done || cur.n0 >= maxLen) break;   // the serial reader stops before opening the file
+        SegmentReader rd(fn, threads);
+        FileData d;
+        while (!done && rd.next(d)) {
+            const size_t nc = d.chunks.size();
+            std::vector<ChunkSummary> sum(nc);
+            parallel(nc, [&](size_t k) {
+                sum[k] = summarize(d.bytes.data() + d.chunks[k].first, d.chunks[k].second - d.chunks[k].first);
+            });
+            // each chunk's starting state; `used` chunks are read, the last of
+            // them only up to the maxLen-th base when the stream fills inside it
+            std::vector<ChunkStart> start(nc);
+            size_t used = 0;
+            bool filled = false;
+            ChunkStart c = cur;
+            for (size_t k = 0; k < nc; ++k) {
+                start[k] = c;
+                used = k + 1;
+                if (sum[k].seqBeforeRec && c.nseq == 0)
+                    throw std::runtime_error("Input file does not appear to be in fasta format\n");
+                const uint64_t take = std::min<uint64_t>(sum[k].nvalid, maxLen - c.n0);
+                if (sum[k].lower && take > 0) anyLower = true;
+                if (c.n0 + take >= maxLen) {
+                    c.n0 += take;
+                    filled = true;
+                    break;
+                }
+                if (sum[k].hasValid) {
+                    c.havePrev = true;
+                    c.prevSeq = c.nseq + sum[k].lastRec - 1;
+                    c.prevPos = sum[k].lastRec ? sum[k].lastPos : c.recPos + sum[k].lastPos;
+                }
+                c.recPos = sum[k].nrec ? sum[k].tailPos : c.recPos + sum[k].tailPos;
+                c.nseq += sum[k].nrec;
                 c.n0 += take;
-                filled = true;
+            }
+            codes.resize((size_t)((c.n0 + 31) / 32), 0);
+            if (anyLower) lower.resize((size_t)((c.n0 + 63) / 64), 0);
+            std::vector<ChunkOut> out(used);
+            parallel(used, [&](size_t k) {
+                out[k] = pack_chunk(d.bytes.data() + d.chunks[k].first, d.chunks[k].second - d.chunks[k].first,
+                                    start[k], maxLen, codes.data(), anyLower ? lower.data() : nullptr);
+            });
+            for (size_t k = 0; k < used; ++k) {
+                s.runStart.insert(s.runStart.end(), out[k].runStart.begin(), out[k].runStart.end());
+                s.runSeq.insert(s.runSeq.end(), out[k].runSeq.begin(), out[k].runSeq.end());
+                s.runPos.insert(s.runPos.end(), out[k].runPos.begin(), out[k].runPos.end());
+                for (std::string& nm : out[k].names) s.names.push_back(std::move(nm));
+            }
+            if (filled) {
+                // records are counted up to the stop point only
+                cur = start[used - 1];
+                cur.nseq += out[used - 1].nrec;
+                cur.n0 += out[used - 1].nbases;
+                done = true;
                 break;
             }
-            if (sum[k].hasValid) {
-                c.havePrev = true;
-                c.prevSeq = c.nseq + sum[k].lastRec - 1;
-                c.prevPos = sum[k].lastRec ? sum[k].lastPos : c.recPos + sum[k].lastPos;
-            }
-            c.recPos = sum[k].nrec ? sum[k].tailPos : c.recPos + sum[k].tailPos;
-            c.nseq += sum[k].nrec;
-            c.n0 += take;
+            cur = c;
         }
-        codes.resize((size_t)((c.n0 + 31) / 32), 0);
-        if (anyLower) lower.resize((size_t)((c.n0 + 63) / 64), 0);
-        std::vector<ChunkOut> out(used);
-        parallel(used, [&](size_t k) {
-            out[k] = pack_chunk(d.bytes.data() + d.chunks[k].first, d.chunks[k].second - d.chunks[k].first,
-                                start[k], maxLen, codes.data(), anyLower ? lower.data() : nullptr);
-        });
-        for (size_t k = 0; k < used; ++k) {
-            s.runStart.insert(s.runStart.end(), out[k].runStart.begin(), out[k].runStart.end());
-            s.runSeq.insert(s.runSeq.end(), out[k].runSeq.begin(), out[k].runSeq.end());
-            s.runPos.insert(s.runPos.end(), out[k].runPos.begin(), out[k].runPos.end());
-            for (std::string& nm : out[k].names) s.names.push_back(std::move(nm));
-        }
-        if (filled) {
-            // records are counted up to the stop point only
-            cur = start[used - 1];
-            cur.nseq += out[used - 1].nrec;
-            cur.n0 += out[used - 1].nbases;
-            break;
-        }
-        cur = c;
     }
     s.n = cur.n0;
     s.nseq = cur.nseq;

@@ -1,6 +1,7 @@
 """The parallel FASTA reader against the oracle's serial filter (CPU).
 
-The reader is built into a small driver with 5-byte minimum chunks, so chunk
+The reader is built into a small driver with 5-byte minimum chunks and
+23-byte file segments (shorter than many lines), so chunk and segment
 boundaries fall on headers, CRLF pairs, N-runs, lowercase runs and the maxLen
 stop; every output (stream bytes, run table, record names, record count) must
 equal the oracle's FASTA filter (sequence.cpp:142-293 restated); record
@@ -18,7 +19,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 @pytest.fixture(scope="module")
 def harness(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("fasta") / "fasta_harness")
-    subprocess.run(["g++", "-O1", "-std=c++17", "-pthread", "-DBLAMM_FASTA_MIN_CHUNK=5", "-o", out,
+    subprocess.run(["g++", "-O1", "-std=c++17", "-pthread", "-DBLAMM_FASTA_MIN_CHUNK=5", "-DBLAMM_FASTA_SEGMENT=23", "-o", out,
                     os.path.join(REPO, "tests", "fasta_harness.cpp"),
                     os.path.join(REPO, "blamm_amd", "csrc", "host", "fasta.cpp")], check=True)
     return out
@@ -93,3 +94,15 @@ def test_parallel_reader_matches_oracle(harness, orc, tmp_path, seed):
             # too); records there reach no output, so counts are compared
             # only for untruncated streams
             assert g[1] == w[1], (seed, max_len, "record count", g[1], w[1])
+
+
+def test_reader_stops_when_full(harness, orc, tmp_path):
+    """Once maxLen bases are packed the reader stops: a later file is not
+    opened (a missing one is no error), as in the serial reader."""
+    fn = str(tmp_path / "a.fa")
+    with open(fn, "wb") as f:
+        f.write(b">r1\n" + b"ACGT" * 40 + b"\n>r2\n" + b"GGCC" * 10 + b"\n")
+    missing = str(tmp_path / "missing.fa")
+    g = run_harness(harness, [fn, missing], 100)
+    assert g[0] == 100 and g[2] == (b"ACGT" * 25)
+    assert run_harness(harness, [fn, missing], 1000) == ("error",)
