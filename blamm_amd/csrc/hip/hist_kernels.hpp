@@ -52,6 +52,9 @@ struct HistArgs {
 
 // LDS histogram budget per workgroup (32 columns x up to 512 bins).
 constexpr uint32_t kHistLdsBytes = 64 * 1024;
+// Undecided windows staged per wave before exact binning (8 KiB per block
+// next to the 64 KiB histogram).
+constexpr int kHistStage = 128;
 
 // ScoreHistogram::addObservation's bin (motif.h:96-102) of an FP32 score:
 // float subtract and IEEE division, then int() as the reference's x86-64
@@ -220,7 +223,7 @@ __device__ __forceinline__ uint32_t hist_range(const HistArgs& a, Stage& st, uin
                                              rnext);
             const uint64_t m = __ballot(defer != 0u);
             if (m) {
-                if (st.n > (uint32_t)(kStage - 64)) exact += hist_exact_stage<LDS>(a, st, hl, blk0, lane);
+                if (st.n > (uint32_t)(kHistStage - 64)) exact += hist_exact_stage<LDS>(a, st, hl, blk0, lane);
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 if (defer) st.buf[st.n + rank] = make_uint4(qlane, slot[b], defer, 0u);
@@ -241,7 +244,7 @@ template <bool MASK, bool LDS>
 __global__ __launch_bounds__(kThreads, 4) void hist_filter_kernel(HistArgs a)
 {
     extern __shared__ uint32_t hist_lds[];
-    __shared__ uint4 stage_lds[kWavesPerBlock][kStage];
+    __shared__ uint4 stage_lds[kWavesPerBlock][kHistStage];
     __shared__ uint32_t item_s;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Stage st{stage_lds[wave], 0u, nullptr};

@@ -111,8 +111,11 @@ __host__ __device__ constexpr uint32_t group_pack(uint32_t blk0, uint32_t nch, u
 // candidates per scan), so each wave collects up to kStage candidates in its
 // own LDS slice and reserves global space with one atomic per flush.  Only
 // the owning wave touches its slice; LDS operations of one wave complete in
-// program order, so no barrier is needed.
-constexpr int kStage = 128;
+// program order, so no barrier is needed.  256 entries (16 KiB per block)
+// keep the filter block at 80 KiB of LDS with its lo fragments, two blocks
+// per CU, and halve the reservations of 128 (measured: -0.5% per scan of
+// 100 Mbp x 2,000 columns).
+constexpr int kStage = 256;
 
 struct Stage {
     uint4* buf;        // this wave's LDS slice
