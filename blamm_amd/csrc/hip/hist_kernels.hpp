@@ -183,10 +183,11 @@ __device__ __forceinline__ uint32_t hist_range(const HistArgs& a, Stage& st, uin
         hoff[b] = LDS ? (uint32_t)(b * kBf16Cols + (lane & 31)) * a.nbins : 0u;
     }
     const uint32_t off = (lane & 31) + 2 * (lane >> 5);   // first base of the lane's window, 0..33
-    const uint32_t woff = off >> 5;
-    const uint32_t sh = 2 * (off & 31), sh_inv = 63 - sh;
-    const uint64_t* __restrict__ pk = a.packed + woff;
-    const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + woff;
+    // pk[t]: dwords 2t + off/16 and the next, so the window of tile t starts
+    // at bit sh < 32 of pk[t]
+    const uint32_t sh = 2 * (off & 15);
+    const u64_a4* __restrict__ pk = (const u64_a4*)((const uint32_t*)a.packed + (off >> 4));
+    const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + (off >> 5);
 
     // run cursor: first run start after the current tile's first window
     uint32_t jr, rnext;
@@ -202,7 +203,7 @@ __device__ __forceinline__ uint32_t hist_range(const HistArgs& a, Stage& st, uin
     }
     for (uint32_t t = t0; t < t1; t += kWavesPerBlock) {
         const uint32_t q0 = t * kBf16Tile;
-        const OneHotSrc bits = onehot_src<NCH>(pk[t], pk[t + 1], sh, sh_inv);
+        const OneHotSrc bits = onehot_src<NCH>(pk[t], pk[t + 1], sh);
         const uint32_t mb = MASK ? funnel32(mk[t], mk[t + 1], off & 31) : 0u;
         f32x16 acc[NB];
         bf16_mfma_tile<NCH, NB, MASK>(acc, bh, bl, bits, mb);

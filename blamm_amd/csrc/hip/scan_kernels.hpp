@@ -215,47 +215,39 @@ constexpr int kGroup = 4;
 // The one-hot operand of chunk j for the lane's window: bf16 1.0 (0x3F80) in
 // lane b of four per base, built in registers (no memory latency; the VALU
 // work overlaps earlier MFMAs); a lowercase base gets an all-zero row.
-// Its source, the lane's bases as shift counts: byte j of p0 / p1 holds 16 x the base
-// at chunk j's first / second position of the lane (bits 4-5 of the byte, the
-// rest zero), so the position's one-hot pair of dwords is 0x3F80 shifted by
-// that byte; v_lshlrev_b64 reads only the low six bits of its count, so a
-// chunk costs one 32-bit shift (none for chunks 0 and 4) and one 64-bit shift
-// per position.
-// Dword [h] covers chunks 4h..4h+3.
+// Its source, the lane's bases as shift counts: byte j of dword 0 / dword 1
+// of p[h] holds 16 x the base at chunk 4h+j's first / second position of the
+// lane (bits 4-5 of the byte, the rest zero), so the position's one-hot pair
+// of dwords is 0x3F80 shifted by that byte.  v_lshlrev_b64 reads only the low
+// six bits of its count, so a chunk costs one 64-bit shift of p[h] (none for
+// chunks 0 and 4) and one 64-bit shift per position.
 struct OneHotSrc {
-    uint32_t p0[2], p1[2];
+    uint64_t p[2];
 };
 
-// Bits [s, s + 32) of the 128-bit value hi:lo (s < 64).
-__device__ __forceinline__ uint32_t funnel_lo32(uint64_t lo, uint64_t hi, uint32_t s)
-{
-    const uint32_t d0 = s < 32 ? (uint32_t)lo : (uint32_t)(lo >> 32);
-    const uint32_t d1 = s < 32 ? (uint32_t)(lo >> 32) : (uint32_t)hi;
-    return __builtin_amdgcn_alignbit(d1, d0, s & 31u);
-}
+// Packed words as loaded by a lane: the lane's pointer is offset to the
+// dword holding its window's first base, which is 4-byte aligned only.
+typedef uint64_t u64_a4 __attribute__((aligned(4)));
 
-// The shift counts of the lane's window whose first base is at bit s of
-// the packed word pair (lo, hi); up to 4 chunks need only 32 bits of it.
+// Shift counts of the lane's window whose first base is at bit s (< 32) of
+// the dword pair lo; bases 16..31 of the window continue in the low dword of
+// hi.  Up to 4 chunks need only 32 bits.
 template <int NCH>
-__device__ __forceinline__ OneHotSrc onehot_src(uint64_t lo, uint64_t hi, uint32_t s, uint32_t s_inv)
+__device__ __forceinline__ OneHotSrc onehot_src(uint64_t lo, uint64_t hi, uint32_t s)
 {
+    constexpr uint32_t M = 0x30303030u;
     OneHotSrc r;
-    if constexpr (NCH <= 4) {
-        const uint32_t bits = funnel_lo32(lo, hi, s);
-        r.p0[0] = (bits << 4) & 0x30303030u;
-        r.p1[0] = (bits << 2) & 0x30303030u;
-        r.p0[1] = r.p1[1] = 0u;
-        // an empty asm (no instruction) keeps the compiler from folding the
-        // masks back into every per-position extraction
-        asm("" : "+v"(r.p0[0]), "+v"(r.p1[0]));
+    const uint32_t b0 = funnel32((uint32_t)lo, (uint32_t)(lo >> 32), s);
+    r.p[0] = ((uint64_t)((b0 << 2) & M) << 32) | ((b0 << 4) & M);
+    // an empty asm (no instruction) keeps the compiler from folding the
+    // masks back into every per-chunk extraction
+    asm("" : "+v"(r.p[0]));
+    if constexpr (NCH > 4) {
+        const uint32_t b1 = funnel32((uint32_t)(lo >> 32), (uint32_t)hi, s);
+        r.p[1] = ((uint64_t)((b1 << 2) & M) << 32) | ((b1 << 4) & M);
+        asm("" : "+v"(r.p[1]));
     } else {
-        const uint64_t bits = funnel64(lo, hi, s, s_inv);
-        const uint64_t a = (bits << 4) & 0x3030303030303030ull, b = (bits << 2) & 0x3030303030303030ull;
-        r.p0[0] = (uint32_t)a;
-        r.p0[1] = (uint32_t)(a >> 32);
-        r.p1[0] = (uint32_t)b;
-        r.p1[1] = (uint32_t)(b >> 32);
-        asm("" : "+v"(r.p0[0]), "+v"(r.p0[1]), "+v"(r.p1[0]), "+v"(r.p1[1]));
+        r.p[1] = 0;
     }
     return r;
 }
@@ -263,8 +255,17 @@ __device__ __forceinline__ OneHotSrc onehot_src(uint64_t lo, uint64_t hi, uint32
 template <bool MASK>
 __device__ __forceinline__ bf16x8 onehot_chunk(const OneHotSrc& src, uint32_t mb, int j)
 {
-    uint64_t o0 = 0x3F80ull << ((src.p0[j >> 2] >> (8 * (j & 3))) & 63u);
-    uint64_t o1 = 0x3F80ull << ((src.p1[j >> 2] >> (8 * (j & 3))) & 63u);
+    uint64_t c = src.p[j >> 2];
+    if (j & 3) {
+        // the shift amount goes through an empty asm so that the compiler,
+        // not knowing it is below 32, keeps one v_lshrrev_b64 instead of
+        // splitting it into two dword shifts
+        uint32_t sa = 8 * (j & 3);
+        asm("" : "+s"(sa));
+        c >>= sa;
+    }
+    uint64_t o0 = 0x3F80ull << ((uint32_t)c & 63u);
+    uint64_t o1 = 0x3F80ull << ((uint32_t)(c >> 32) & 63u);
     if (MASK) {
         if ((mb >> (4 * j)) & 1) o0 = 0;
         if ((mb >> (4 * j + 1)) & 1) o1 = 0;
@@ -402,17 +403,18 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
     }
 
     const uint32_t off = (lane & 31) + 2 * (lane >> 5);   // first base of the lane's window, 0..33
-    const uint32_t woff = off >> 5;
-    const uint32_t sh = 2 * (off & 31), sh_inv = 63 - sh;
-    const uint64_t* __restrict__ pk = a.packed + woff;
-    const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + woff;
+    // pk[t]: dwords 2t + off/16 and the next of the packed stream, so the
+    // window of tile t starts at bit sh < 32 of pk[t]
+    const uint32_t sh = 2 * (off & 15);
+    const u64_a4* __restrict__ pk = (const u64_a4*)((const uint32_t*)a.packed + (off >> 4));
+    const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + (off >> 5);
 #if BLAMM_LAZY_LO == 2
     // hi products of tile t are issued before tile t-1 is finished
     static_assert(NB * NCH <= 2 * kMaxChunks, "lo fragments exceed the wave's LDS slice");
     f32x16 accA[NB], accB[NB];
     OneHotSrc bA, bB{};
     uint32_t mA = 0, mB = 0, m0 = 0, m1 = 0, m2 = 0;
-    bA = onehot_src<NCH>(pk[t0], pk[t0 + 1], sh, sh_inv);
+    bA = onehot_src<NCH>(pk[t0], pk[t0 + 1], sh);
     if (MASK) mA = funnel32(mk[t0], mk[t0 + 1], off & 31);
     bf16_hi_tile<NCH, NB, MASK>(accA, bh, bA, mA);
     uint64_t w0 = pk[t0 + 1], w1 = pk[t0 + 2], w2 = pk[t0 + 3];
@@ -423,11 +425,11 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         const uint64_t w3 = pk[t + 3], w4 = pk[t + 4];
         uint32_t m3 = 0, m4 = 0;
         if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
-        bB = onehot_src<NCH>(w0, w1, sh, sh_inv);
+        bB = onehot_src<NCH>(w0, w1, sh);
         if (MASK) mB = funnel32(m0, m1, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
         bf16_finish_tile<NCH, NB, MASK, true>(a, st, accA, bl, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
-        bA = onehot_src<NCH>(w1, w2, sh, sh_inv);
+        bA = onehot_src<NCH>(w1, w2, sh);
         if (MASK) mA = funnel32(m1, m2, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accA, bh, bA, mA);
         bf16_finish_tile<NCH, NB, MASK, true>(a, st, accB, bl, th, thh, bB, mB, t * kBf16Tile, blk0, lane);
@@ -435,7 +437,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
     if (t < t1) {                   // one tile left: t
-        bB = onehot_src<NCH>(w0, w1, sh, sh_inv);
+        bB = onehot_src<NCH>(w0, w1, sh);
         if (MASK) mB = funnel32(m0, m1, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
         bf16_finish_tile<NCH, NB, MASK, true>(a, st, accA, bl, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
@@ -456,7 +458,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
     for (uint32_t t = t0; t < t1; ++t) {
         const uint64_t c2 = pk[t + 2];
         const uint32_t cm2 = MASK ? mk[t + 2] : 0u;
-        const OneHotSrc bits = onehot_src<NCH>(c0, c1, sh, sh_inv);
+        const OneHotSrc bits = onehot_src<NCH>(c0, c1, sh);
         const uint32_t mb = MASK ? funnel32(cm0, cm1, off & 31) : 0u;
         bf16_hi_tile<NCH, NB, MASK>(acc, bh, bits, mb);
         mfma_result_guard();
@@ -476,7 +478,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         const uint32_t q0m = MASK ? mk[t0] : 0u;
         uint64_t p1 = pk[t0 + 1];
         if (MASK) m0 = mk[t0 + 1];
-        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, onehot_src<NCH>(p0, p1, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, onehot_src<NCH>(p0, p1, sh),
                                       MASK ? funnel32(q0m, m0, off & 31) : 0u);
     }
     uint64_t w0 = pk[t0 + 1], w1 = pk[t0 + 2], w2 = pk[t0 + 3];
@@ -487,17 +489,17 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         const uint64_t w3 = pk[t + 3], w4 = pk[t + 4];
         uint32_t m3 = 0, m4 = 0;
         if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
-        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, onehot_src<NCH>(w0, w1, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, onehot_src<NCH>(w0, w1, sh),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
         bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
-        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, onehot_src<NCH>(w1, w2, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accA, bh, bl, onehot_src<NCH>(w1, w2, sh),
                                       MASK ? funnel32(m1, m2, off & 31) : 0u);
         bf16_epilogue<NB>(a, st, accB, th, t * kBf16Tile, blk0, lane);
         w0 = w2; w1 = w3; w2 = w4;
         if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
     if (t < t1) {                   // one tile left: t
-        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, onehot_src<NCH>(w0, w1, sh, sh_inv),
+        bf16_mfma_tile<NCH, NB, MASK>(accB, bh, bl, onehot_src<NCH>(w0, w1, sh),
                                       MASK ? funnel32(m0, m1, off & 31) : 0u);
         bf16_epilogue<NB>(a, st, accA, th, (t - 1) * kBf16Tile, blk0, lane);
         pendingA = false;
