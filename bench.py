@@ -322,7 +322,7 @@ def precision_leg(args, orc_slice_bases=500_000):
     t0 = time.time()
     ms, recs = synth.config2(seed=2, n_each=args.precision_bases)
     log(f"precision workload: {len(recs)} groups x {args.precision_bases} bases ({time.time() - t0:.1f}s)")
-    res = {"bf16_ms": 0.0, "fp32_ms": 0.0, "groups": [], "identical_hit_sets": True}
+    res = {"bf16_ms": 0.0, "bf16lo_ms": 0.0, "fp32_ms": 0.0, "groups": [], "identical_hit_sets": True}
     total_work = 0.0
     slice_check = None
     for gi, (name, codes, gc) in enumerate(recs):
@@ -336,7 +336,7 @@ def precision_leg(args, orc_slice_bases=500_000):
         d_codes, d_runs = to_device(words, n, runs)
         g = {"group": name, "gc": gc, "bases": n}
         keys = {}
-        for prec, tag in ((ba.PREC_BF16X2, "bf16"), (ba.PREC_FP32, "fp32")):
+        for prec, tag in ((ba.PREC_BF16X2, "bf16"), (ba.PREC_BF16X2_LO, "bf16lo"), (ba.PREC_FP32, "fp32")):
             ds = DeviceScan(P, L, thr, prec)
             nh = ds.step(d_codes, n, d_runs, 1)
             torch.cuda.synchronize()
@@ -351,7 +351,7 @@ def precision_leg(args, orc_slice_bases=500_000):
             if tag == "bf16":
                 g["score_err_bound"] = ds.sc.bounds()[1]
             ds.close()
-        g["identical"] = bool(np.array_equal(keys["bf16"], keys["fp32"]))
+        g["identical"] = bool(np.array_equal(keys["bf16"], keys["fp32"]) and np.array_equal(keys["bf16"], keys["bf16lo"]))
         res["identical_hit_sets"] &= g["identical"]
         total_work += float(n) * len(pwms)
         res["groups"].append(g)
@@ -381,6 +381,7 @@ def precision_leg(args, orc_slice_bases=500_000):
                     f"{len(recs)} groups x {args.precision_bases // 1_000_000} Mbp GC U(0.35,0.50), p=1e-5",
         "value": total_work / (res["bf16_ms"] / 1e3) / 1e9,
         "fp32_value": total_work / (res["fp32_ms"] / 1e3) / 1e9,
+        "lazy_lo_value": total_work / (res["bf16lo_ms"] / 1e3) / 1e9,
         "unit": "Gbp*motifs/s",
         "speedup_vs_fp32": res["fp32_ms"] / res["bf16_ms"],
         "fp64_slice": slice_check,
@@ -639,6 +640,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-lo", action="store_true", help="skip the lazy-lo form of the split-BF16 scan")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--tune-cache", default=os.path.join(os.path.expanduser("~"), ".cache", "blamm_amd", "tune.txt"))
     ap.add_argument("--sections", default="batched,precision,serve,hist",
@@ -679,6 +681,15 @@ def main():
     log("split-bf16 leg")
     bf, bf_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_BF16X2, tune_sample=nbp)
     log(f"split-bf16: {bf}")
+    lo = None
+    if not args.no_lo:
+        log("split-bf16 lazy-lo leg")
+        lo, lo_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_BF16X2_LO, tune_sample=16_000_000)
+        log(f"split-bf16 lazy lo: {lo}")
+        a = np.sort(bf_hits["pos"].astype(np.uint64) * np.uint64(1 << 16) + bf_hits["col"])
+        b = np.sort(lo_hits["pos"].astype(np.uint64) * np.uint64(1 << 16) + lo_hits["col"])
+        lo["identical_hit_set"] = bool(a.size == b.size and np.array_equal(a, b))
+        del lo_hits
     fp = None
     if not args.no_fp32:
         log("fp32 leg")
@@ -702,11 +713,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x2->f32 (+f64 band)",
+        "dtype": "bf16x2 (hi terms on MFMA)->f32 (+f32/f64 rescoring)",
         "data": "synthetic (seeded)",
         "config": {"workload": f"plain-GPU config: {ncols // 2} motifs (L 5-30) fwd+rc = {ncols} columns, "
                                f"{args.bases // 1_000_000} Mbp GC 0.41 CpG 0.2x 0.5% N-runs, p={args.pvalue:g}",
-                   "bases": nbp, "columns": ncols, "pvalue": args.pvalue, "precision": "split-bf16"},
+                   "bases": nbp, "columns": ncols, "pvalue": args.pvalue,
+                   "precision": "split-bf16, hi-only filter"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
                      "kernel": "filter_persistent_kernel<true,false,false> (one launch per step)",
@@ -718,6 +730,11 @@ def main():
     if "host_api" in bf:
         bf["host_api"]["value"] = nbp * ncols / (bf["host_api"]["ms_per_scan"] / 1e3) / 1e9
         bf["host_api"]["value_pinned"] = nbp * ncols / (bf["host_api"]["ms_per_scan_pinned"] / 1e3) / 1e9
+    if lo is not None:
+        lo["value"] = nbp * ncols / (lo["ms_per_step"] / 1e3) / 1e9
+        lach = flops / (lo["filter_ms"] / 1e3) / 1e12
+        lo.update({"roofline_achieved_TFLOPs": lach, "roofline_frac": lach / PEAK_BF16_TFLOPS})
+        line["split_lazy_lo"] = lo
     if fp is not None:
         fval = nbp * ncols / (fp["ms_per_step"] / 1e3) / 1e9
         fach = flops / (fp["filter_ms"] / 1e3) / 1e12

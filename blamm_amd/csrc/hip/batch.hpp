@@ -139,7 +139,7 @@ int batch_grow(blamm_hip_batch* b, DevBuf<T>& buf, size_t want)
 int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
 {
     blamm_hip_ctx* c = b->ctx;
-    const bool bf = c->precision == BLAMM_PREC_BF16X2;
+    const bool bf = prec_bf(c->precision);
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, b->stream, b->counters.p);
     HIP_TRY(hipGetLastError());
     // the batch's column blocks from the library's fragments
@@ -173,7 +173,7 @@ int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
     a.cand_cap = b->cand.n;
     const WorkQueue q{0, 0, 1, 1};
     const ListArgs l{b->desc.p, b->items.p, b->chunks.p};
-    hipError_t e = launch_filter(bf, mask, true, a, q, l, grid, b->stream);
+    hipError_t e = launch_filter(bf, prec_lo(c->precision), mask, true, a, q, l, grid, b->stream);
     if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch filter launch: ") + hipGetErrorString(e));
     FinalizeArgs f = finalize_args(c, b->cand.p, b->cand.n, b->counters.p, b->packed.p, mask ? b->lcmask.p : nullptr,
                                    b->runs.p);
@@ -216,8 +216,9 @@ int batch_launch(blamm_hip_batch* b)
     if (!b->exec[v] || b->exec_buf_gen[v] != b->buf_gen || b->exec_lib_gen[v] != c->lib_gen) {
         if (b->exec[v]) (void)hipGraphExecDestroy(b->exec[v]);
         b->exec[v] = nullptr;
-        const bool bf = c->precision == BLAMM_PREC_BF16X2;
-        const uint32_t grid = (uint32_t)c->num_cu * (uint32_t)persistent_wgs_per_cu(bf, b->mask, true);
+        const bool bf = prec_bf(c->precision);
+        const uint32_t grid =
+            (uint32_t)c->num_cu * (uint32_t)persistent_wgs_per_cu(bf, prec_lo(c->precision), b->mask, true);
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         const int rc = batch_enqueue_kernels(b, b->mask, grid);
         hipGraph_t g = nullptr;
@@ -351,7 +352,7 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     const size_t nslots = (size_t)c->nslots;
     const uint32_t cpb = (uint32_t)c->cols_per_blk;
     const uint32_t ct = std::max<uint32_t>(1, c->tuning.chunk_tiles);
-    const bool bf = c->precision == BLAMM_PREC_BF16X2;
+    const bool bf = prec_bf(c->precision);
     const uint32_t nbmax = std::min<uint32_t>(c->tuning.blocks_per_wave, bf ? kNbMaxBf16 : kNbMaxFp32);
     const int kstep = (int)c->tuning.kpad_positions / 4;
 

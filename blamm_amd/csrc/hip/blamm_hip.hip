@@ -175,14 +175,23 @@ struct blamm_hip_ctx {
 
 namespace {
 
+// Precision modes: FP32, or bf16 with the filter in its hi-only or lazy-lo
+// form (both keep the PWM split into hi and lo terms on the device).
+inline bool prec_bf(int p) { return p == BLAMM_PREC_BF16X2 || p == BLAMM_PREC_BF16X2_LO; }
+inline bool prec_lo(int p) { return p == BLAMM_PREC_BF16X2_LO; }
+
 template <bool BF, bool MASK, bool LIST>
 const void* filter_kernel_ptr()
 {
     return (const void*)filter_persistent_kernel<BF, MASK, LIST>;
 }
 
-const void* filter_kernel(bool bf, bool mask, bool list)
+const void* filter_kernel(bool bf, bool lo, bool mask, bool list)
 {
+    if (bf && lo) {
+        if (mask) return list ? (const void*)filter_lo_kernel<true, true> : (const void*)filter_lo_kernel<true, false>;
+        return list ? (const void*)filter_lo_kernel<false, true> : (const void*)filter_lo_kernel<false, false>;
+    }
     if (bf) {
         if (mask) return list ? filter_kernel_ptr<true, true, true>() : filter_kernel_ptr<true, true, false>();
         return list ? filter_kernel_ptr<true, false, true>() : filter_kernel_ptr<true, false, false>();
@@ -192,19 +201,23 @@ const void* filter_kernel(bool bf, bool mask, bool list)
 }
 
 // Workgroups of the persistent filter resident per CU (register/LDS limits).
-int persistent_wgs_per_cu(bool bf, bool mask, bool list)
+int persistent_wgs_per_cu(bool bf, bool lo, bool mask, bool list)
 {
     int n = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_kernel(bf, mask, list), kThreads, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, filter_kernel(bf, lo, mask, list), kThreads, 0);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-hipError_t launch_filter(bool bf, bool mask, bool list, const FilterArgs& a, const WorkQueue& q, const ListArgs& l,
-                         uint32_t grid, hipStream_t s)
+hipError_t launch_filter(bool bf, bool lo, bool mask, bool list, const FilterArgs& a, const WorkQueue& q,
+                         const ListArgs& l, uint32_t grid, hipStream_t s)
 {
 #define BLAMM_LAUNCH(B, M, L) \
     hipLaunchKernelGGL((filter_persistent_kernel<B, M, L>), dim3(grid), dim3(kThreads), 0, s, a, q, l)
-    if (bf) {
+#define BLAMM_LAUNCH_LO(M, L) hipLaunchKernelGGL((filter_lo_kernel<M, L>), dim3(grid), dim3(kThreads), 0, s, a, q, l)
+    if (bf && lo) {
+        if (mask) { if (list) BLAMM_LAUNCH_LO(true, true); else BLAMM_LAUNCH_LO(true, false); }
+        else { if (list) BLAMM_LAUNCH_LO(false, true); else BLAMM_LAUNCH_LO(false, false); }
+    } else if (bf) {
         if (mask) { if (list) BLAMM_LAUNCH(true, true, true); else BLAMM_LAUNCH(true, true, false); }
         else { if (list) BLAMM_LAUNCH(true, false, true); else BLAMM_LAUNCH(true, false, false); }
     } else {
@@ -212,6 +225,7 @@ hipError_t launch_filter(bool bf, bool mask, bool list, const FilterArgs& a, con
         else { if (list) BLAMM_LAUNCH(false, false, true); else BLAMM_LAUNCH(false, false, false); }
     }
 #undef BLAMM_LAUNCH
+#undef BLAMM_LAUNCH_LO
     return hipGetLastError();
 }
 
@@ -235,7 +249,7 @@ __global__ void store_count_kernel(const unsigned long long* counters, uint64_t*
 int check_tuning(const blamm_tuning& t, int precision)
 {
     if (t.chunk_tiles < 1 || t.chunk_tiles > (1u << 20)) return fail(BLAMM_ERR_ARG, "tuning: chunk_tiles outside 1..2^20");
-    const uint32_t nbmax = precision == BLAMM_PREC_BF16X2 ? kNbMaxBf16 : kNbMaxFp32;
+    const uint32_t nbmax = prec_bf(precision) ? kNbMaxBf16 : kNbMaxFp32;
     if (t.blocks_per_wave != 1 && t.blocks_per_wave != 2 && t.blocks_per_wave != 4)
         return fail(BLAMM_ERR_ARG, "tuning: blocks_per_wave must be 1, 2 or 4");
     if (t.blocks_per_wave > nbmax)
@@ -249,7 +263,7 @@ int check_tuning(const blamm_tuning& t, int precision)
 // upload it.
 int relayout(blamm_hip_ctx* c)
 {
-    const bool bf = c->precision == BLAMM_PREC_BF16X2;
+    const bool bf = prec_bf(c->precision);
     const int ncols = c->ncols, ldp = c->ldp;
     const int* lengths = c->lib_len.data();
     const float* P = c->lib_P.data();
@@ -575,7 +589,7 @@ int blamm_hip_set_library(blamm_hip_ctx* c, const float* P, int ldp, int ncols,
     if (!c) return fail(BLAMM_ERR_ARG, "set_library: null context");
     if (!P || !lengths || !thresholds || ncols <= 0)
         return fail(BLAMM_ERR_ARG, "set_library: empty library");
-    if (precision != BLAMM_PREC_FP32 && precision != BLAMM_PREC_BF16X2)
+    if (precision != BLAMM_PREC_FP32 && !prec_bf(precision))
         return fail(BLAMM_ERR_ARG, "set_library: unknown precision");
     int lmax = 0;
     for (int j = 0; j < ncols; ++j) {
@@ -601,7 +615,7 @@ int blamm_hip_set_library(blamm_hip_ctx* c, const float* P, int ldp, int ncols,
     c->lib_P.assign(P, P + (size_t)ncols * ldp);
     c->lib_len.assign(lengths, lengths + ncols);
     c->lib_thr.assign(thresholds, thresholds + ncols);
-    const uint32_t nbmax = precision == BLAMM_PREC_BF16X2 ? kNbMaxBf16 : kNbMaxFp32;
+    const uint32_t nbmax = prec_bf(precision) ? kNbMaxBf16 : kNbMaxFp32;
     if (c->tuning.blocks_per_wave > nbmax) c->tuning.blocks_per_wave = nbmax;
     return relayout(c);
 }
@@ -666,7 +680,7 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     if (c->cand.n == 0) HIP_TRY(c->cand.reserve(1u << 20));
 
-    const bool bf = c->precision == BLAMM_PREC_BF16X2;
+    const bool bf = prec_bf(c->precision);
     const bool mask = d_lcmask != nullptr;
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, c->counters.p);
     HIP_TRY(hipGetLastError());
@@ -683,8 +697,9 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
             return fail(BLAMM_ERR_ARG, "scan: work queue too long");
         const ListArgs l{nullptr, nullptr, nullptr};
         const uint32_t grid =
-            (uint32_t)c->num_cu * (uint32_t)persistent_wgs_per_cu(bf, mask, false) * (uint32_t)c->filter_rounds;
-        hipError_t e = launch_filter(bf, mask, false, a, q, l, grid, s);
+            (uint32_t)c->num_cu * (uint32_t)persistent_wgs_per_cu(bf, prec_lo(c->precision), mask, false) *
+            (uint32_t)c->filter_rounds;
+        hipError_t e = launch_filter(bf, prec_lo(c->precision), mask, false, a, q, l, grid, s);
         if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("filter launch: ") + hipGetErrorString(e));
     }
     HIP_TRY(hipEventRecord(c->ev[1], s));

@@ -101,7 +101,7 @@ int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     if (!c) return fail(BLAMM_ERR_ARG, "hist: null context");
     if (!c->have_lib) return fail(BLAMM_ERR_STATE, "hist: no library uploaded");
     if (c->pending) return fail(BLAMM_ERR_STATE, "hist: a scan is in flight");
-    if (c->precision != BLAMM_PREC_BF16X2)
+    if (!prec_bf(c->precision))
         return fail(BLAMM_ERR_ARG, "hist: the library must be uploaded in the split-BF16 form");
     if (!min_score || !max_score || !counts) return fail(BLAMM_ERR_ARG, "hist: null histogram range or counts");
     if (nbins < 1 || nbins > (1u << 24)) return fail(BLAMM_ERR_ARG, "hist: nbins outside 1..2^24");

@@ -22,7 +22,7 @@ std::string tune_key(const blamm_hip_ctx* c, uint64_t nbases, bool mask)
     int lg = 0;
     while ((1ull << (lg + 1)) <= std::max<uint64_t>(nbases, 1)) ++lg;
     std::ostringstream os;
-    os << c->arch << "/cu" << c->num_cu << "/" << (c->precision == BLAMM_PREC_BF16X2 ? "bf16x2" : "fp32") << "/cols"
+    os << c->arch << "/cu" << c->num_cu << "/" << (c->precision == BLAMM_PREC_BF16X2 ? "bf16x2" : c->precision == BLAMM_PREC_BF16X2_LO ? "bf16x2lo" : "fp32") << "/cols"
        << c->ncols << "/len" << std::hex << h << std::dec << "/n2^" << lg << (mask ? "/mask" : "");
     return os.str();
 }
@@ -88,7 +88,7 @@ int blamm_hip_autotune(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t pack
     if (tune_lookup(cache_path, key, &best, &best_ms)) {
         BLAMM_TRY(blamm_hip_set_tuning(c, &best));
     } else {
-        const uint32_t nbmax = c->precision == BLAMM_PREC_BF16X2 ? kNbMaxBf16 : kNbMaxFp32;
+        const uint32_t nbmax = prec_bf(c->precision) ? kNbMaxBf16 : kNbMaxFp32;
         const uint32_t ntiles = (uint32_t)((nbases + kBf16Tile - 1) / kBf16Tile);
         best_ms = 1e300;
         // layouts at the default window tile, then window tiles for the best layout

@@ -110,7 +110,7 @@ void scan_usage()
                  "  -rt\t--relthreshold\tset the minimal relative score [0..1] for a motif occurrence (default = 0.95)\n"
                  "  -pt\t--pthreshold\tcompute the motif score threshold from p-value [0..1] (default = 1E-4)\n"
                  "  -t\t--numthreads\tset the number of host threads [default = #cores]\n"
-                 "  -p\t--precision\tbf16x2 (default) or fp32\n"
+                 "  -p\t--precision\tbf16x2 (default), bf16x2lo (lo products on the MFMA too) or fp32\n"
                  "  -d\t--device\tHIP device index [default = 0]\n\n"
                  " [file_options]\n"
                  "  -H\t--histdir\tdirectory where the histogram file(s) are stored [default = .]\n"
@@ -517,7 +517,8 @@ int scan_main(int argc, char** argv)
             const std::string p(argv[++i]);
             if (p == "fp32") precision = BLAMM_PREC_FP32;
             else if (p == "bf16x2") precision = BLAMM_PREC_BF16X2;
-            else throw std::runtime_error("precision must be fp32 or bf16x2");
+            else if (p == "bf16x2lo") precision = BLAMM_PREC_BF16X2_LO;
+            else throw std::runtime_error("precision must be fp32, bf16x2 or bf16x2lo");
         } else if ((a == "-d" || a == "--device") && more) {
             device = std::atoi(argv[++i]);
         } else if ((a == "-T" || a == "--tune") && more) {
@@ -564,7 +565,7 @@ int scan_main(int argc, char** argv)
         int ndev = 0;
         blamm_hip_device_count(&ndev);
         if (ndev == 0) throw std::runtime_error("HIP error: no devices found. Aborting...");
-        std::cout << "Using 1 GPU device (" << (precision == BLAMM_PREC_FP32 ? "fp32" : "split-bf16")
+        std::cout << "Using 1 GPU device (" << (precision == BLAMM_PREC_FP32 ? "fp32" : precision == BLAMM_PREC_BF16X2_LO ? "split-bf16, lazy lo" : "split-bf16")
                   << " MFMA) and " << numThreads << " host thread(s)" << std::endl;
         hip_check(blamm_hip_create(device, &hc.ctx), "device setup");
     };

@@ -55,7 +55,11 @@ enum blamm_status {
 
 enum blamm_precision {
     BLAMM_PREC_FP32 = 0,      /* FP32 MFMA, FP32 accumulate */
-    BLAMM_PREC_BF16X2 = 1     /* PWM split into bf16 hi+lo, BF16 MFMA, FP32 accumulate */
+    BLAMM_PREC_BF16X2 = 1,    /* PWM split into bf16 hi+lo; the filter multiplies the hi
+                                 terms on BF16 MFMA (FP32 accumulate) and keeps every
+                                 window the proven hi-only band cannot rule out */
+    BLAMM_PREC_BF16X2_LO = 2  /* PWM split into bf16 hi+lo; tile blocks the hi terms do
+                                 not rule out also get their lo products on BF16 MFMA */
 };
 
 /* One occurrence.  pos: position in the packed stream of the scan call;
@@ -103,8 +107,8 @@ int blamm_hip_destroy(blamm_hip_ctx *ctx);
  *               score >= threshold (kernel.cu:29, pwmscan.cpp:117-119)
  *   precision:  enum blamm_precision
  * Every weight must be finite.  The hit set is that of a left-to-right FP64
- * sum of the window's weights compared with (double)threshold in both
- * precisions; the filter bands that make it so are per column and proven. */
+ * sum of the window's weights compared with (double)threshold in every
+ * precision; the filter bands that make it so are per column and proven. */
 int blamm_hip_set_library(blamm_hip_ctx *ctx, const float *P, int ldp, int ncols,
                           const int *lengths, const float *thresholds, int precision);
 

@@ -19,7 +19,7 @@ from helpers import hit_set, library, stream_case
 
 pytestmark = pytest.mark.gpu
 
-PRECS = [ba.PREC_BF16X2, ba.PREC_FP32]
+PRECS = [ba.PREC_BF16X2, ba.PREC_BF16X2_LO, ba.PREC_FP32]
 
 
 def say(*a):
@@ -187,7 +187,7 @@ def test_tunings_agree_with_oracle(scanner, orc, prec):
     scanner.set_library(P, lengths, thr, prec)
     ref = orc.scan_fp64(text, runs, pwms, thr)
     packed, lower = ba.pack(text)
-    nbs = [1, 2] if prec == ba.PREC_BF16X2 else [1, 2, 4]
+    nbs = [1, 2] if prec != ba.PREC_FP32 else [1, 2, 4]
     try:
         for nb in nbs:
             for kp in (4, 8):

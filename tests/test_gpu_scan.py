@@ -19,7 +19,7 @@ from helpers import (EXAMPLE, assert_same_occurrences, copy_example, hit_set, li
 
 pytestmark = pytest.mark.gpu
 
-PRECS = [ba.PREC_BF16X2, ba.PREC_FP32]
+PRECS = [ba.PREC_BF16X2, ba.PREC_BF16X2_LO, ba.PREC_FP32]
 
 
 def say(*a):
@@ -155,7 +155,7 @@ def test_repeated_scans_are_identical(scanner, orc, prec):
     scanner.set_library(P, lengths, thr, prec)
     want = hit_set(orc.scan_fp64(text, runs, pwms, thr))
     packed, lower = ba.pack(text)
-    nbs = [1, 2] if prec == ba.PREC_BF16X2 else [1, 2, 4]
+    nbs = [1, 2] if prec != ba.PREC_FP32 else [1, 2, 4]
     bad = []
     try:
         for rep in range(2):
@@ -283,7 +283,7 @@ def test_precisions_agree_mid_size(scanner, orc):
         say(f"mid-size prec={prec}")
         scanner.set_library(P, lengths, thr, prec)
         out[prec] = scanner.scan(packed, sc.size, runs, None, cap=1 << 22)
-    assert hit_set(out[PRECS[0]]) == hit_set(out[PRECS[1]])
+    assert hit_set(out[PRECS[0]]) == hit_set(out[PRECS[1]]) == hit_set(out[PRECS[2]])
     a, b = 1_500_000, 1_700_000
     text = synth.CODE2BASE[sc[a:b]].tobytes()
     sub_runs = [0] + [int(r) - a for r in runs if a < r < b]
@@ -378,6 +378,7 @@ def test_full_size_plain_gpu_config(scanner, orc):
     h = out[ba.PREC_BF16X2]
     key = lambda x: np.sort(x["pos"].astype(np.uint64) * np.uint64(4096) + x["col"].astype(np.uint64))
     assert np.array_equal(key(h), key(out[ba.PREC_FP32]))
+    assert np.array_equal(key(h), key(out[ba.PREC_BF16X2_LO]))
     assert len(h) > 10_000_000
     lmax = int(lengths.max())
     rng = np.random.default_rng(5)

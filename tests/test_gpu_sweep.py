@@ -71,7 +71,7 @@ def test_sweep_scan_and_hist(scanner, orc, seed):
     packed, lower = ba.pack(text)
     want = orc.scan_fp64(text, runs, pwms, thr, threads=4)
     want_set = set(zip(want["pos"].tolist(), want["col"].tolist()))
-    for prec in (ba.PREC_BF16X2, ba.PREC_FP32):
+    for prec in (ba.PREC_BF16X2, ba.PREC_BF16X2_LO, ba.PREC_FP32):
         scanner.set_library(P, lengths, thr, prec)
         got = scanner.scan(packed, len(text), runs, lower)
         got_set = set(zip(got["pos"].tolist(), got["col"].tolist()))
