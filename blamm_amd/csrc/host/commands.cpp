@@ -23,6 +23,7 @@
 #include <thread>
 
 #include "../../../include/blamm_hip.h"
+#include "format_g6.hpp"
 #include "host.hpp"
 
 namespace blamm {
@@ -137,8 +138,7 @@ inline void put_u64(std::string& s, uint64_t v)
 inline void put_score(std::string& s, float f)
 {
     char b[32];
-    auto r = std::to_chars(b, b + sizeof(b), (double)f, std::chars_format::general, 6);
-    s.append(b, r.ptr);
+    s.append(b, put_g6(b, f));
 }
 
 struct ColumnInfo {
