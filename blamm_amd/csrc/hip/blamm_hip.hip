@@ -804,6 +804,12 @@ int scan_staged(blamm_hip_ctx* c, const uint64_t* packed, const uint64_t* lcmask
             return BLAMM_OK;
         }
         if (rc != BLAMM_ERR_OVERFLOW) return rc;
+        if (n == 0 && c->stats.candidates > dcap) {
+            // the candidate scratch overflowed (and was grown): size the hits
+            // for as many as there are records, which usually holds them all,
+            // so the retry is the last run
+            dcap = std::min<uint64_t>(cap_limit, c->stats.candidates);
+        }
         if (n > dcap) {   // hit buffer too small
             if (n > cap_limit) {
                 *nhits = n;
