@@ -684,7 +684,7 @@ def main():
     lo = None
     if not args.no_lo:
         log("split-bf16 lazy-lo leg")
-        lo, lo_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_BF16X2_LO, tune_sample=16_000_000)
+        lo, lo_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_BF16X2_LO, tune_sample=nbp)
         log(f"split-bf16 lazy lo: {lo}")
         a = np.sort(bf_hits["pos"].astype(np.uint64) * np.uint64(1 << 16) + bf_hits["col"])
         b = np.sort(lo_hits["pos"].astype(np.uint64) * np.uint64(1 << 16) + lo_hits["col"])
