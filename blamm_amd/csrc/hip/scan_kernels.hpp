@@ -212,8 +212,9 @@ constexpr int kGroup = 4;
 // 4 bases), B = PWM chunk (16 x 32 columns), D = 32x32 FP32 scores.
 // Lane l: A rows/ B cols index l%32, K half h = l/32 holds positions 2h, 2h+1
 // of the chunk; D lane l holds column l%32, rows 8(r/4) + 4h + r%4.
-// Each wave keeps NB column blocks (NCH chunks, hi and lo) in registers and
-// streams window tiles through them.
+// Each wave keeps the hi terms of NB column blocks (NCH chunks) in registers
+// and streams window tiles through them (the lazy-lo form keeps the lo terms
+// in its LDS slice; the histogram kernel keeps both in registers).
 // ---------------------------------------------------------------------------
 // The one-hot operand of chunk j for the lane's window: bf16 1.0 (0x3F80) in
 // lane b of four per base, built in registers (no memory latency; the VALU
