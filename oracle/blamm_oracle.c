@@ -8,36 +8,53 @@
  * Citations are to the reference tree (biointec/blamm) as file:line.
  *
  * Pinning status (what fixes each function besides this restatement; the
- * checks live in tests/test_oracle_*.py):
+ * checks live in tests/pinning.py, run by tests/test_oracle_pinning.py):
  *   filter_fasta          hand-built FASTA cases with closed-form runs/positions
- *                         (N-runs, CRLF, lowercase, empty records, dict truncation)
- *   pfm2pwm               closed form with the host libm log2f, rc identity
- *                         W_rc[p][b] == W[L-1-p][3-b] (bitwise)
- *   max/min score         brute force over all 4^L words for small L
- *   score_cutoff          hand-written histograms with closed-form cutoffs
- *   load_jaspar/load_cb   the reference's example/motifs.jaspar (counts checked
- *                         against the file text) and hand-built files
+ *                         (N-runs, CRLF, lowercase, empty records, dict
+ *                         truncation)                      check_fasta_cases
+ *   pfm2pwm               closed form with the host libm log2f (bitwise), rc
+ *                         identity W_rc[p][b] == W[L-1-p][3-b]
+ *                                 check_pwm_values / _bitwise / _revcompl
+ *   max/min score         brute force over all 4^L words  check_min_max_bruteforce
+ *   score_cutoff          hand-written histograms with closed-form cutoffs,
+ *                         float-coordinate cases           check_cutoff_*
+ *   load_jaspar           the reference's example/motifs.jaspar, counts typed
+ *                         from the file; hand-built layouts ("[1" swallowing a
+ *                         count, "3]", blank lines, no final newline)
+ *                                 check_load_jaspar_example / _layouts
+ *   load_cb, is_permutation, base_name
+ *                         hand-built cluster-buster file (names with blanks,
+ *                         "__NN", "__", "X_1", "__5", "_5") with and without
+ *                         permutations; a table of names
+ *                                 check_load_cluster_buster / check_permutation_names
+ *   dict writer           closed form on hand-built FASTA (case-folded counts,
+ *                         group and file order, names, CRLF) check_dict_closed_form
+ *   theoretical_hist      exact: designs where every float step is exact, bins
+ *                         and counts fixed by hand (store-not-add collision,
+ *                         round-half-away grid, float count product) and by an
+ *                         exact-arithmetic brute force on 40 random dyadic
+ *                         designs                          check_hist_dyadic_exact
+ *                         plus support/count invariants on random PWMs
  *   scan_fp64             pure-Python brute force on tiny inputs; rc symmetry;
- *                         -at thresholds placed exactly on achievable sums (>=)
+ *                         -at thresholds placed exactly on achievable sums (>=);
+ *                         lowercase contributes 0         check_scan_*
  *   scan_sgemm            equals scan_fp64 except inside the FP32 error band;
  *                         independent of MATRIX_S_W/S_H and of the tiling
- *   theoretical_hist      closed form for L = 1, 2 (convolution of the
- *                         background) and the store-not-add bin quirk
- *                         (motif.h:109-115) on a constructed motif
- *   dict writer           closed-form ACGT counts of hand-built FASTA
+ *                                 tests/test_oracle_pinning.py
  *   empirical_hist        closed forms: L = 1 (bin counts = base counts),
  *                         dyadic PWMs whose sums and bin edges are exact
  *                         (pure-Python brute force), total = number of valid
- *                         windows, lowercase-only windows, a window whose FP32
- *                         left-to-right sum and exact sum fall on either side
- *                         of a bin edge, width 0 (every window in bin 0)
+ *                         windows, a window whose FP32 left-to-right sum and
+ *                         exact sum fall on either side of a bin edge, width 0
+ *                                 check_ehist_*
  * Planted mistakes: compiling with -DORACLE_MUTATE_<name> swaps one rule for a
  * plausible wrong one (GT, NO_BG_SWAP, LOWER_COUNTS, NO_RUN_CHECK,
  * CUTOFF_DOUBLE, HIST_ADD, RUNS_IGNORE_RECORD, HIST_ROUND, HIST_NO_RUN_CHECK,
- * HIST_FP64); tests/test_oracle_mutations.py
- * shows each is caught by a pinning check.
- * The reference itself is not built here: it needs the CMake-generated
- * config.h and an external BLAS, so no run of the
+ * HIST_FP64, HIST_RINT, HIST_COUNT_DOUBLE, JASPAR_ONE_SKIP, PERM_NEEDS_DIGIT,
+ * DICT_UPPER_ONLY, DICT_FILE_ORDER); tests/test_oracle_mutations.py shows
+ * each is caught by a pinning check.
+ * The reference itself is not built here: its sources include the
+ * CMake-generated config.h and need an external BLAS, so no run of the
  * reference binary pins any of the above.
  */
 #define _GNU_SOURCE
@@ -323,7 +340,11 @@ static void load_jaspar(const char *fn, motif_t **v, size_t *n, size_t *cap)
             if (!line) continue;
             const char *s = line;
             /* skip two tokens ("A" and "[") */
+#ifdef ORACLE_MUTATE_JASPAR_ONE_SKIP
+            for (int t = 0; t < 1; t++) {   /* wrong: leaves "[" in front of the counts */
+#else
             for (int t = 0; t < 2; t++) {
+#endif
                 while (*s && is_ws((unsigned char)*s)) s++;
                 while (*s && !is_ws((unsigned char)*s)) s++;
             }
@@ -374,6 +395,9 @@ static int is_permutation(const char *name)
     long i = (long)strlen(name) - 1;
     for (; i >= 0; i--) if (!isdigit((unsigned char)name[i])) break;
     if (i < 1) return 0;
+#ifdef ORACLE_MUTATE_PERM_NEEDS_DIGIT
+    if (i == (long)strlen(name) - 1) return 0;   /* wrong: "X__" is a permutation too */
+#endif
     return name[i - 1] == '_' && name[i] == '_';
 }
 
@@ -444,6 +468,35 @@ static motif_t *add_rc(motif_t *v, size_t n)
     free(v);
     return r;
 }
+
+int oracle_load_motifs(const char *fn, int perms, size_t *n, char ***names, int **lengths, uint64_t **pfms)
+{
+    FILE *f = fopen(fn, "rb");
+    if (!f) return -1;
+    fclose(f);
+    size_t k;
+    motif_t *m = load_motifs(fn, perms, &k);
+    size_t tot = 0;
+    for (size_t i = 0; i < k; i++) tot += (size_t)m[i].L;
+    *names = xmalloc((k ? k : 1) * sizeof(char *));
+    *lengths = xmalloc((k ? k : 1) * sizeof(int));
+    *pfms = xmalloc((tot ? tot : 1) * 4 * sizeof(uint64_t));
+    size_t at = 0;
+    for (size_t i = 0; i < k; i++) {
+        (*names)[i] = m[i].name;
+        (*lengths)[i] = m[i].L;
+        if (m[i].L) memcpy(*pfms + 4 * at, m[i].pfm, (size_t)m[i].L * 4 * sizeof(uint64_t));
+        at += (size_t)m[i].L;
+        free(m[i].pfm);
+    }
+    free(m);
+    *n = k;
+    return 0;
+}
+
+int oracle_is_permutation(const char *name) { return is_permutation(name); }
+
+char *oracle_base_name(const char *name) { return base_name(name); }
 
 /* ------------------------------------------------------------------------ */
 /* histograms (motif.cpp:47-132, 151-192; hist.cpp:162-175)                 */
@@ -517,7 +570,11 @@ int oracle_theoretical_hist(const float *pwm, int L, const float *bg,
     for (int p = 0; p < L; p++) {
         int mn = 0, mx = 0;
         for (int k = 0; k < 4; k++) {
+#ifdef ORACLE_MUTATE_HIST_RINT
+            pi[4 * p + k] = (int)rintf(a * pwm[4 * p + k] + b);   /* wrong: ties to even */
+#else
             pi[4 * p + k] = (int)roundf(a * pwm[4 * p + k] + b);
+#endif
             if (k == 0 || pi[4 * p + k] < mn) mn = pi[4 * p + k];
             if (k == 0 || pi[4 * p + k] > mx) mx = pi[4 * p + k];
         }
@@ -567,8 +624,12 @@ int oracle_theoretical_hist(const float *pwm, int L, const float *bg,
          * keys arrive in ascending int order so a repeat is adjacent */
         if (have && score == lastScore) continue;
         have = 1; lastScore = score;
+#ifdef ORACLE_MUTATE_HIST_COUNT_DOUBLE
+        size_t cnt = (size_t)((double)maxLength * (double)cur[i]);   /* wrong: product in double */
+#else
         float c = (float)maxLength * cur[i];
         size_t cnt = (size_t)c;
+#endif
         int bin = (int)((score - minS) / width);
         if (bin < 0) bin = 0;
         if (bin > (int)numBins - 1) bin = (int)numBins - 1;
@@ -1101,7 +1162,9 @@ static void set_insert(char ***v, size_t *n, const char *s)
     for (size_t i = 0; i < *n; i++) if (!strcmp((*v)[i], s)) return;
     *v = xrealloc(*v, (*n + 1) * sizeof(char *));
     (*v)[(*n)++] = xstrdup(s);
+#ifndef ORACLE_MUTATE_DICT_FILE_ORDER   /* wrong when defined: files kept in manifest order */
     qsort(*v, *n, sizeof(char *), cmp_str);
+#endif
 }
 
 static species_t *load_dict(const char *fn, size_t *nout)
@@ -1181,7 +1244,12 @@ static int cmd_dict(int argc, char **argv)
         if (n < 0) die("%s", err);
         uint64_t cnt[4] = {0, 0, 0, 0};
         for (int64_t i = 0; i < n; i++) {
+#ifdef ORACLE_MUTATE_DICT_UPPER_ONLY
+            int ch = (unsigned char)stream[i];   /* wrong: lowercase bases not counted */
+            if (ch >= 'a') continue;
+#else
             int ch = toupper((unsigned char)stream[i]);
+#endif
             cnt[ch == 'A' ? 0 : ch == 'C' ? 1 : ch == 'G' ? 2 : 3]++;
         }
         /* sequence names: first token after '>' (sequence.cpp:155-161) */

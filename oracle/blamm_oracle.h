@@ -100,6 +100,21 @@ int oracle_empirical_hist(const char *stream, uint64_t n, const uint64_t *runSta
                           const float *minScore, const float *maxScore, int ncols,
                           size_t numBins, uint64_t qend, int nthreads, uint64_t *counts);
 
+/* MotifContainer::load (motif.cpp:323-338): jaspar when the name ends in
+ * ".jaspar" (loadJasparMotifs, motif.cpp:370-437), else cluster-buster
+ * (loadCBMotifs, motif.cpp:340-368); sorted by length (stable here, the
+ * reference's std::sort is not); permutation motifs (names ending in "__" and
+ * digits, motif.h:272-281) dropped unless perms != 0.  On return *names is an
+ * array of *n malloc'd strings, *lengths their lengths and *pfms the
+ * concatenated L x 4 counts (all malloc'd; free with oracle_free).
+ * Returns 0, or -1 when the file cannot be opened. */
+int oracle_load_motifs(const char *fn, int perms, size_t *n, char ***names, int **lengths, uint64_t **pfms);
+
+/* Motif::isPermutation / getBaseName (motif.h:256-281); base_name returns a
+ * malloc'd string. */
+int oracle_is_permutation(const char *name);
+char *oracle_base_name(const char *name);
+
 /* Command line entry (dict / hist / scan), same syntax as the reference. */
 int oracle_main(int argc, char **argv);
 

@@ -79,7 +79,53 @@ class Oracle:
         L.oracle_empirical_hist.argtypes = [ctypes.c_char_p, ctypes.c_uint64, _u64p, ctypes.c_uint64, _f32p,
                                             ctypes.c_int, _i32p, _f32p, _f32p, ctypes.c_int, ctypes.c_size_t,
                                             ctypes.c_uint64, ctypes.c_int, _u64p]
+        L.oracle_load_motifs.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t),
+                                         ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.POINTER(ctypes.c_void_p)]
+        L.oracle_is_permutation.argtypes = [ctypes.c_char_p]
+        L.oracle_base_name.argtypes = [ctypes.c_char_p]
+        L.oracle_base_name.restype = ctypes.c_void_p
+        L.oracle_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
         self.L = L
+
+    def load_motifs(self, path: str, perms: bool = True):
+        """[(name, L x 4 counts)] as MotifContainer::load leaves them (sorted
+        by length; permutations kept only with perms)."""
+        n = ctypes.c_size_t()
+        names, lens, pfms = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        if self.L.oracle_load_motifs(path.encode(), int(perms), ctypes.byref(n), ctypes.byref(names),
+                                     ctypes.byref(lens), ctypes.byref(pfms)) != 0:
+            raise RuntimeError(f"cannot open {path}")
+        k = n.value
+        name_ptrs = ctypes.cast(names, ctypes.POINTER(ctypes.c_void_p))
+        len_arr = ctypes.cast(lens, ctypes.POINTER(ctypes.c_int))
+        out, at = [], 0
+        pf = ctypes.cast(pfms, ctypes.POINTER(ctypes.c_uint64))
+        for i in range(k):
+            nm = ctypes.string_at(name_ptrs[i]).decode("latin-1")
+            L = len_arr[i]
+            counts = np.array([pf[4 * at + j] for j in range(4 * L)], np.uint64).reshape(L, 4)
+            at += L
+            out.append((nm, counts))
+            self.L.oracle_free(name_ptrs[i])
+        for ptr in (names, lens, pfms):
+            self.L.oracle_free(ptr)
+        return out
+
+    def is_permutation(self, name: str) -> bool:
+        return bool(self.L.oracle_is_permutation(name.encode()))
+
+    def base_name(self, name: str) -> str:
+        ptr = self.L.oracle_base_name(name.encode())
+        out = ctypes.string_at(ptr).decode()
+        self.L.oracle_free(ptr)
+        return out
+
+    def main(self, args) -> int:
+        """The oracle's command line (dict / hist / scan) in this process; it
+        exits the process on input errors, so give it valid inputs only."""
+        argv = (ctypes.c_char_p * (len(args) + 1))(b"blamm_oracle", *[a.encode() for a in args])
+        return self.L.oracle_main(len(args) + 1, argv)
 
     def log2f(self, x: float) -> float:
         return self.L.oracle_log2f(x)

@@ -142,7 +142,9 @@ def check_cutoff_float_coordinate(orc) -> None:
 
 # ---- theoretical histogram (motif.cpp:151-192, hist.cpp:162-175) ---------------
 # Every grid score s sits on the boundary of bin s, so which of bins s-1, s
-# it lands in is decided by float rounding; these checks hold for either.
+# it lands in is decided by float rounding; the two invariant checks below
+# hold for either.  check_hist_dyadic_exact fixes the bin and the count of
+# every grid score on designs where every float operation is exact.
 
 def _grid(w, bg, nb):
     """Integer grid scores and their probabilities (exact enumeration)."""
@@ -193,6 +195,242 @@ def check_hist_store_not_add(orc) -> None:
     dist = _grid(w, bg, 2)
     assert counts.sum() <= 2 * int(1_000_000 * max(dist.values())) + 2, counts
     assert counts.sum() < 990_000, counts   # adding would keep ~all of the 1e6 mass
+
+
+def _round_half_away(x: Fraction) -> int:
+    """std::round: nearest integer, halves away from zero."""
+    f = math.floor(abs(x) + Fraction(1, 2))
+    return f if x >= 0 else -f
+
+
+def _exact_theoretical(w, bg, nb, max_length):
+    """The theoretical histogram from its definition in exact arithmetic, for
+    designs where every float step of the reference is exact: the affine map
+    a = nb / (max - min), b = -a * min / L rounds each weight to a grid
+    integer (halves away from zero), the grid sums' probabilities are the
+    convolution of the background, a grid score s maps back to the score
+    (s - L b) / a, whose bin coordinate (score - min) / width is exact; bins
+    are clamped to [0, nb - 1], a later (higher) score overwrites an earlier
+    one in its bin, and the count is floor(max_length * probability)."""
+    L = len(w)
+    W = [[Fraction(float(x)) for x in row] for row in w]
+    B = [Fraction(float(x)) for x in bg]
+    mn = sum(min(r) for r in W)
+    mx = sum(max(r) for r in W)
+    a = Fraction(nb) / (mx - mn)
+    b = -a * mn / L
+    g = [[_round_half_away(a * x + b) for x in row] for row in W]
+    dist = {}
+    for word in itertools.product(range(4), repeat=L):
+        s = sum(g[p][k] for p, k in enumerate(word))
+        pr = Fraction(1)
+        for k in word:
+            pr *= B[k]
+        dist[s] = dist.get(s, 0) + pr
+    width = (mx - mn) / nb
+    counts = [0] * nb
+    # max_length enters the product as a float (size_t * float converts the
+    # integer); the designs keep that product exact
+    mf = Fraction(float(np.float32(max_length)))
+    for s in sorted(dist):
+        score = (s - L * b) / a
+        coord = (score - mn) / width
+        j = min(nb - 1, max(0, math.floor(coord)))
+        prod = mf * dist[s]
+        assert Fraction(float(np.float32(float(prod)))) == prod, "design leaves the float product inexact"
+        counts[j] = math.floor(prod)
+    return counts, float(mn), float(mx)
+
+
+# Designs whose float steps are all exact: weights on a quarter grid, ranges
+# and backgrounds dyadic, nb / range and min / L dyadic, max_length * p exact
+# in float (or, for the last case, the float product differs from the double
+# one).  (weights, background, bins, max_length, expected counts by hand)
+HIST_DYADIC = [
+    # a = 1, b = 0: grid = weights; s = 8 clamps into bin 7 and overwrites
+    # s = 7 (store, not add); counts truncate 15.625, 31.25, 46.875, 62.5 ...
+    ([[0, 1, 2, 4], [0, 1, 3, 4]], [0.125, 0.125, 0.25, 0.5], 8, 1000,
+     [15, 31, 46, 62, 156, 187, 125, 250]),
+    # a = 2, b = 1: a w + b = -0.5, 2.5 and 4.5 round away from zero
+    # (-1, 3, 5; ties-to-even would give 0, 2, 4)
+    ([[-1, -0.75, 0.75, 1], [0, 0.25, 1.75, 2]], [0.125, 0.125, 0.25, 0.5], 8, 1000,
+     [31, 31, 0, 0, 281, 93, 0, 562]),
+    # (float)100000007 = 100000008: the float product gives 12500001 where
+    # the double product truncates 12500000.875 to 12500000
+    ([[0, 1, 2, 4]], [0.125, 0.125, 0.25, 0.5], 4, 100000007,
+     [12500001, 12500001, 25000002, 50000004]),
+]
+
+
+def check_hist_dyadic_exact(orc) -> None:
+    """Bin and count of every grid score fixed by hand (HIST_DYADIC) and by
+    exact-arithmetic brute force on random dyadic designs (L = 1, 2, 4;
+    8 to 64 bins; clamping, collisions and half-integer grid points)."""
+    for w, bg, nb, M, want in HIST_DYADIC:
+        w = np.array(w, np.float32)
+        counts, mn, mx = orc.theoretical_hist(w, np.array(bg, np.float32), nb, M)
+        assert counts.tolist() == want, (w.tolist(), counts.tolist(), want)
+        ex, emn, emx = _exact_theoretical(w, bg, nb, M)
+        assert ex == want and (mn, mx) == (emn, emx), (ex, want)
+    rng = np.random.default_rng(77)
+    done = 0
+    while done < 40:
+        L = int(rng.choice([1, 2, 4]))
+        k = int(rng.integers(1, 4))              # range = 2^k
+        nb = int(rng.choice([8, 16, 32, 64]))
+        if Fraction(nb, 2 ** k).denominator != 1 and nb < 2 ** k:
+            continue
+        # per position: min and max on the quarter grid, the spans summing to 2^k
+        spans = rng.multinomial(4 * 2 ** k - L, np.ones(L) / L) + 1   # quarters, each >= 1
+        rows = []
+        for p in range(L):
+            lo = int(rng.integers(-8, 5))
+            hi = lo + int(spans[p])
+            mid = rng.integers(lo, hi + 1, size=2)
+            rows.append([lo / 4, hi / 4, float(mid[0]) / 4, float(mid[1]) / 4])
+            rng.shuffle(rows[-1])
+        mn = sum(Fraction(min(r)).limit_denominator(4) for r in rows)
+        if (mn / L * Fraction(nb, 2 ** k)).denominator > 2 ** 20:
+            continue
+        bgq = rng.multinomial(16 - 4, np.ones(4) / 4) + 1          # sixteenths, each >= 1
+        bg = [q / 16 for q in bgq]
+        w = np.array(rows, np.float32)
+        counts, _, _ = orc.theoretical_hist(w, np.array(bg, np.float32), nb, 1000)
+        ex, _, _ = _exact_theoretical(w, bg, nb, 1000)
+        assert counts.tolist() == ex, (rows, bg, nb, counts.tolist(), ex)
+        done += 1
+
+
+# ---- motif loaders (motif.cpp:323-437, motif.h:256-281) ---------------------------
+
+# The reference's example/motifs.jaspar (tests/golden/example), typed from the
+# file: rows A, C, G, T, one count per position.
+EXAMPLE_JASPAR = {
+    "MA0001.2": [[22, 16, 25, 27, 0, 0, 82, 40, 56, 35, 65, 25, 64, 0, 33, 52, 45, 21],
+                 [29, 9, 8, 16, 92, 79, 1, 4, 0, 0, 1, 4, 0, 0, 14, 5, 23, 24],
+                 [34, 29, 4, 13, 0, 0, 2, 3, 1, 0, 4, 3, 28, 92, 15, 7, 13, 26],
+                 [10, 41, 58, 39, 3, 16, 10, 48, 38, 60, 25, 63, 3, 3, 33, 31, 14, 24]],
+    "MA0002.2": [[287, 234, 123, 57, 0, 87, 0, 17, 10, 131, 500],
+                 [496, 485, 1072, 0, 75, 127, 0, 42, 400, 463, 158],
+                 [696, 467, 149, 7, 1872, 70, 1987, 1848, 251, 81, 289],
+                 [521, 814, 656, 1936, 53, 1716, 13, 93, 1339, 1325, 1053]],
+}
+
+
+def _example_jaspar_path():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "example", "motifs.jaspar")
+
+
+def check_load_jaspar_example(orc) -> None:
+    """The reference's example file: both motifs, shortest first (sorted by
+    length, motif.cpp:436), counts as typed from the file (every position of
+    MA0001.2 sums to 95)."""
+    got = orc.load_motifs(_example_jaspar_path())
+    assert [n for n, _ in got] == ["MA0002.2", "MA0001.2"], [n for n, _ in got]
+    for name, pfm in got:
+        want = np.array(EXAMPLE_JASPAR[name], np.uint64).T
+        assert pfm.shape == want.shape and np.array_equal(pfm, want), name
+    assert set(got[1][1].sum(axis=1).tolist()) == {95}
+
+
+JASPAR_LAYOUTS = (">M1 first motif\nA [ 1 2 3 ]\nC [ 4 5 6 ]\nG [ 7 8 9 ]\nT [ 10 11 12 ]\n\n"
+                  ">M2\tx\nA  [1 2 3 ]\nC  [4 5 6]\nG  [ 7 8 9]\nT  [10 11 12 ]\n"
+                  ">M3\nA [ 5 ]\nC [ 0 ]\nG [ 0 ]\nT [ 95 ]")
+
+
+def check_load_jaspar_layouts(orc) -> None:
+    """Header names are the first token without '>'; each count row skips two
+    tokens ("A" and "[", so "[1" swallows a count) and reads counts until the
+    first token that is not one ("3]" still reads 3); the motif length is the
+    A row's count; blank lines between motifs and a missing final newline
+    are accepted."""
+    want = {"M1": [[1, 4, 7, 10], [2, 5, 8, 11], [3, 6, 9, 12]],
+            "M2": [[2, 5, 7, 11], [3, 6, 8, 12]],
+            "M3": [[5, 0, 0, 95]]}
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "m.jaspar")
+        with open(fn, "w") as f:
+            f.write(JASPAR_LAYOUTS)
+        got = orc.load_motifs(fn)
+    assert [n for n, _ in got] == ["M3", "M2", "M1"], [n for n, _ in got]
+    for name, pfm in got:
+        assert pfm.tolist() == want[name], (name, pfm.tolist())
+
+
+# (name, is a permutation, base name) by motif.h:256-281: a trailing run of
+# digits (possibly empty) preceded by "__", at index >= 1
+PERMUTATION_NAMES = [
+    ("MA0001.2", False, "MA0001.2"), ("M__1", True, "M"), ("M__", True, "M"), ("M_1", False, "M_1"),
+    ("__5", True, ""), ("_5", False, "_5"), ("5", False, "5"), ("", False, ""), ("A__B__12", True, "A__B"),
+    ("A__12x", False, "A__12x"), ("M___7", True, "M_"), ("__", True, ""), ("X__007", True, "X"),
+]
+
+
+def check_permutation_names(orc) -> None:
+    for name, perm, base in PERMUTATION_NAMES:
+        assert orc.is_permutation(name) == perm, name
+        assert orc.base_name(name) == base, name
+
+
+CLUSTER_BUSTER = (">A1 first motif\n1 2 3 4\n5 6 7 8\n\n>A1__12\n0 0 0 9\n>B__\n1 1 1 1\n2 2 2 2\n3 3 3 3\n"
+                  ">X_1\n9 8 7 6\n>__5\n4 4 4 4\n>_5\n3 3 3 3\n>Q99\n1 0 0 0\n")
+
+
+def check_load_cluster_buster(orc) -> None:
+    """Cluster-buster files (any name not ending in .jaspar): a '>' line names
+    a motif with the whole rest of the line; each other non-empty line adds
+    one position of four counts; blank lines are skipped; hist loads the
+    motifs without permutations (perms = false), scan with them."""
+    text = CLUSTER_BUSTER
+    want = {"A1 first motif": [[1, 2, 3, 4], [5, 6, 7, 8]], "A1__12": [[0, 0, 0, 9]],
+            "B__": [[1, 1, 1, 1], [2, 2, 2, 2], [3, 3, 3, 3]], "X_1": [[9, 8, 7, 6]], "__5": [[4, 4, 4, 4]],
+            "_5": [[3, 3, 3, 3]], "Q99": [[1, 0, 0, 0]]}
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "motifs.cb")
+        with open(fn, "w") as f:
+            f.write(text)
+        for perms, names in ((True, set(want)), (False, {"A1 first motif", "X_1", "_5", "Q99"})):
+            got = orc.load_motifs(fn, perms)
+            assert {n for n, _ in got} == names and len(got) == len(names), (perms, [n for n, _ in got])
+            lens = [p.shape[0] for _, p in got]
+            assert lens == sorted(lens), lens
+            for name, pfm in got:
+                assert pfm.tolist() == want[name], name
+
+
+# ---- dict writer (dict.cpp:53-115; species.cpp:32-61, 84-103, 176-188) -------------
+
+def check_dict_closed_form(orc) -> None:
+    """`dict` on hand-built FASTA: groups in order of first appearance in the
+    manifest, each group's files in lexicographic order (a std::set), ACGT
+    counts case-folded over valid bases only, TOT_SEQ_LENGTH their sum,
+    sequence names = the first token after '>' in file order (CRLF and leading
+    blanks stripped), blank manifest lines skipped."""
+    with tempfile.TemporaryDirectory() as d:
+        mf, want = write_dict_case(d)
+        assert orc.main(["dict", mf]) == 0
+        got = open(mf + ".dict").read()
+    assert got == want, got
+
+
+def write_dict_case(d):
+    """Hand-built FASTA files and manifest under d; returns the manifest path
+    and the .dict text the reference writes for it (closed form)."""
+    f0, f1, f2 = (os.path.join(d, x) for x in ("f0.fa", "f1.fa", "f2.fa"))
+    with open(f1, "wb") as f:
+        f.write(b">s1 first\nACGTacgtNNAC\nGT\n>s2\nnnnn\n> s3 x\nAAAA\n")
+    with open(f0, "wb") as f:
+        f.write(b">z9\r\nCCCC\r\n")
+    with open(f2, "wb") as f:
+        f.write(b">only\nggggtttt\n")
+    mf = os.path.join(d, "sequences.mf")
+    with open(mf, "w") as f:
+        f.write(f"grpB\t{f1}\n\ngrpA\t{f2}\ngrpB\t{f0}\n")
+    want = (f"NUM_SPECIES\t2\nSPECIES\tgrpB\nNUM_FASTA_FILES\t2\n{f0}\n{f1}\nTOT_SEQ_LENGTH\t20\n"
+            f"NUCL_COUNT_ACGT\t7\t7\t3\t3\nNUM_SEQUENCES\t4\nz9\ns1\ns2\ns3\n"
+            f"SPECIES\tgrpA\nNUM_FASTA_FILES\t1\n{f2}\nTOT_SEQ_LENGTH\t8\nNUCL_COUNT_ACGT\t0\t0\t4\t4\n"
+            f"NUM_SEQUENCES\t1\nonly\n")
+    return mf, want
 
 
 # ---- FASTA filtering (sequence.cpp:142-293) -------------------------------------

@@ -89,6 +89,38 @@ def test_dict_and_hist_match_oracle(tmp_path, variant):
             assert open(a / name).read() == open(b / name).read(), name
 
 
+def test_dict_matches_closed_form(tmp_path):
+    """`blamm dict` on the hand-built case whose .dict text is fixed by hand
+    (tests/pinning.py write_dict_case)."""
+    import pinning
+    mf, want = pinning.write_dict_case(str(tmp_path))
+    run_blamm(["dict", mf], str(tmp_path))
+    assert open(mf + ".dict").read() == want
+
+
+@pytest.mark.parametrize("kind", ["jaspar", "cb"])
+def test_hist_on_pinned_motif_files(tmp_path, kind):
+    """`blamm hist` reads the hand-built motif files of the loader pins like
+    the oracle: the same .dat files (one per non-permutation motif, named by
+    the loaded names) with the same contents."""
+    import pinning
+    a, b = tmp_path / "prod", tmp_path / "orc"
+    fn = "motifs.jaspar" if kind == "jaspar" else "motifs.cb"
+    for d in (a, b):
+        d.mkdir()
+        copy_example(str(d))
+        (d / fn).write_text(pinning.JASPAR_LAYOUTS if kind == "jaspar" else pinning.CLUSTER_BUSTER)
+    run_blamm(["dict", "sequences.mf"], str(a))
+    run_oracle(["dict", "sequences.mf"], str(b))
+    run_blamm(["hist", fn, "sequences.mf"], str(a))
+    run_oracle(["hist", fn, "sequences.mf"], str(b))
+    dats = sorted(p.name for p in a.glob("hist_*.dat"))
+    assert dats == sorted(p.name for p in b.glob("hist_*.dat"))
+    assert len(dats) == (3 if kind == "jaspar" else 4) * 2   # two groups in the example manifest
+    for name in dats:
+        assert open(a / name).read() == open(b / name).read(), name
+
+
 def test_pwm_matches_oracle(orc):
     rng = np.random.default_rng(17)
     for _ in range(50):

@@ -17,6 +17,12 @@ MUTATIONS = {
     "ORACLE_MUTATE_HIST_ROUND": "check_ehist_dyadic_bruteforce",
     "ORACLE_MUTATE_HIST_NO_RUN_CHECK": "check_ehist_total_windows",
     "ORACLE_MUTATE_HIST_FP64": "check_ehist_fp32_order",
+    "ORACLE_MUTATE_HIST_RINT": "check_hist_dyadic_exact",
+    "ORACLE_MUTATE_HIST_COUNT_DOUBLE": "check_hist_dyadic_exact",
+    "ORACLE_MUTATE_JASPAR_ONE_SKIP": "check_load_jaspar_example",
+    "ORACLE_MUTATE_PERM_NEEDS_DIGIT": "check_load_cluster_buster",
+    "ORACLE_MUTATE_DICT_UPPER_ONLY": "check_dict_closed_form",
+    "ORACLE_MUTATE_DICT_FILE_ORDER": "check_dict_closed_form",
 }
 
 
