@@ -34,7 +34,7 @@ BLAMM_ERR_STATE = -5
 PREC_FP32 = 0
 PREC_BF16X2 = 1
 PREC_BF16X2_LO = 2
-MAX_MOTIF_LEN = 32
+MAX_MOTIF_LEN = 64
 
 HIT_DTYPE = np.dtype([("pos", "<u8"), ("col", "<u4"), ("score", "<f4")])
 

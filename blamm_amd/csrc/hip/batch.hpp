@@ -395,7 +395,8 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
             size_t e = i;
             while (e < nchs.size() && nchs[e] == nchs[i]) ++e;
             size_t bb = i;
-            for (uint32_t nb = nbmax; nb >= 1; nb /= 2)
+            const uint32_t nbg = nchs[i] > kWideChunks ? 1u : nbmax;
+            for (uint32_t nb = nbg; nb >= 1; nb /= 2)
                 while (e - bb >= nb) {
                     gr.push_back({blk_first + (uint32_t)bb, (uint32_t)nchs[i], nb});
                     bb += nb;

@@ -19,8 +19,7 @@
 #include <vector>
 
 #include "../../../include/blamm_hip.h"
-#include "scan_kernels.hpp"
-#include "hist_kernels.hpp"
+#include "instances.hpp"
 
 using namespace blamm;
 
@@ -169,6 +168,8 @@ struct blamm_hip_ctx {
     // last scan
     bool pending = false;
     uint64_t pending_hit_cap = 0;
+    uint64_t pending_cand_cap = 0;            // record capacity the filter was launched with
+    unsigned long long* pending_counters = nullptr;
     hipStream_t pending_stream = nullptr;
     blamm_scan_stats stats{};
 };
@@ -291,14 +292,18 @@ int relayout(blamm_hip_ctx* c)
         nch = std::min(kMaxChunks, (nch + kstep - 1) / kstep * kstep);
         blk_nch.push_back(nch);
     }
-    // Groups: runs of equal chunk count cut into nbmax blocks, the remainder
-    // into smaller powers of two, so no padding block is scored.
+    // Groups: runs of equal chunk count cut into nbmax blocks (one block for
+    // motifs longer than 4 kWideChunks positions), the remainder into
+    // smaller powers of two, so no padding block is scored.
     std::vector<GroupInfo> groups;
+    int max_nch = 1;
     for (size_t i = 0; i < blocks.size();) {
         size_t e = i;
         while (e < blocks.size() && blk_nch[e] == blk_nch[i]) ++e;
         size_t b = i;
-        for (uint32_t nb = nbmax; nb >= 1; nb /= 2)
+        max_nch = std::max(max_nch, blk_nch[i]);
+        const uint32_t nbg = blk_nch[i] > kWideChunks ? 1u : nbmax;
+        for (uint32_t nb = nbg; nb >= 1; nb /= 2)
             while (e - b >= nb) {
                 groups.push_back({(uint32_t)b, (uint32_t)blk_nch[i], nb});
                 b += nb;
@@ -360,7 +365,9 @@ int relayout(blamm_hip_ctx* c)
             // roundoff 2^-22 to allow for truncating accumulation.
             const double nterms = bf ? 32.0 * nch : 16.0 * nch;
             const double band = 2.0 * (rep_err + gamma_n(nterms, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
-            const double nterms_any = bf ? 32.0 * kMaxChunks : 16.0 * kMaxChunks;
+            // a batch may place the column in a block of any chunk count the
+            // library has
+            const double nterms_any = bf ? 32.0 * max_nch : 16.0 * max_nch;
             s_band_any[slot] =
                 2.0 * (rep_err + gamma_n(nterms_any, std::ldexp(1.0, -22)) * sum_split + fp64_err) + 1e-30;
             // hi-only score H': the lo terms it leaves out are bounded by
@@ -601,6 +608,7 @@ int blamm_hip_set_library(blamm_hip_ctx* c, const float* P, int ldp, int ncols,
             return fail(BLAMM_ERR_ARG, "set_library: NaN threshold in column " + std::to_string(j));
     }
     if (ldp < 4 * lmax) return fail(BLAMM_ERR_ARG, "set_library: ldp < 4 * max motif length");
+    if (c->pending) return fail(BLAMM_ERR_STATE, "set_library: a scan is in flight (call scan_finish first)");
     for (int j = 0; j < ncols; ++j)
         for (int k = 0; k < 4 * lengths[j]; ++k)
             if (!std::isfinite(P[(size_t)j * ldp + k]))
@@ -653,16 +661,21 @@ int blamm_hip_set_tuning(blamm_hip_ctx* c, const blamm_tuning* t)
 int blamm_hip_reserve(blamm_hip_ctx* c, uint64_t n)
 {
     if (!c) return fail(BLAMM_ERR_ARG, "reserve: null context");
+    if (c->pending) return fail(BLAMM_ERR_STATE, "reserve: a scan is in flight (call scan_finish first)");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(c->cand.reserve(n));
     return BLAMM_OK;
 }
 
-int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t packed_words,
-                          const uint64_t* d_lcmask, uint64_t mask_words, uint64_t nbases,
-                          const uint64_t* d_run_starts, uint64_t nruns, blamm_hit* d_hits,
-                          uint64_t cap, uint64_t* d_nhits, void* stream)
+}  // extern "C"
+
+namespace {
+
+// Argument checks shared by the device-buffer scan entry points.
+int check_scan_args(const blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t packed_words, const uint64_t* d_lcmask,
+                    uint64_t mask_words, uint64_t nbases, const uint64_t* d_run_starts, uint64_t nruns,
+                    const blamm_hit* d_hits, uint64_t cap)
 {
     if (!c) return fail(BLAMM_ERR_ARG, "scan: null context");
     if (!c->have_lib) return fail(BLAMM_ERR_STATE, "scan: no library uploaded");
@@ -676,18 +689,23 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     if (nruns > 0 && !d_run_starts) return fail(BLAMM_ERR_ARG, "scan: null run starts");
     if (nruns >= (1ull << 32)) return fail(BLAMM_ERR_ARG, "scan: too many runs");
     if (cap > 0 && !d_hits) return fail(BLAMM_ERR_ARG, "scan: null hit buffer");
-    HIP_TRY(hipSetDevice(c->device));
-    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    if (c->cand.n == 0) HIP_TRY(c->cand.reserve(1u << 20));
+    return BLAMM_OK;
+}
 
+// Enqueue counter clear, filter, finalize (and the hit-count copy) on s, with
+// the filter records in cand[0..cand_cap) and the counters in `counters`.
+int enqueue_scan(blamm_hip_ctx* c, const uint64_t* d_packed, const uint64_t* d_lcmask, uint64_t nbases,
+                 const uint64_t* d_run_starts, uint64_t nruns, blamm_hit* d_hits, uint64_t cap, uint64_t* d_nhits,
+                 uint4* cand, uint64_t cand_cap, unsigned long long* counters, hipStream_t s)
+{
     const bool bf = prec_bf(c->precision);
     const bool mask = d_lcmask != nullptr;
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, c->counters.p);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counters);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[0], s));
     if (nbases > 0) {
         const uint32_t ntiles = (uint32_t)((nbases + kBf16Tile - 1) / kBf16Tile);   // 32-window tiles
-        const FilterArgs a = filter_args(c, d_packed, d_lcmask, c->slot_thr.p, c->cand.p, c->cand.n, c->counters.p);
+        const FilterArgs a = filter_args(c, d_packed, d_lcmask, c->slot_thr.p, cand, cand_cap, counters);
         WorkQueue q{};
         q.ntiles = ntiles;
         q.chunk_tiles = std::max<uint32_t>(1, std::min<uint32_t>(c->tuning.chunk_tiles, ntiles));
@@ -704,7 +722,7 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     }
     HIP_TRY(hipEventRecord(c->ev[1], s));
     {
-        FinalizeArgs f = finalize_args(c, c->cand.p, c->cand.n, c->counters.p, d_packed, d_lcmask, d_run_starts);
+        FinalizeArgs f = finalize_args(c, cand, cand_cap, counters, d_packed, d_lcmask, d_run_starts);
         f.nbases = (uint32_t)nbases;
         f.nruns = (uint32_t)nruns;
         f.hits = d_hits;
@@ -714,13 +732,33 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     }
     HIP_TRY(hipEventRecord(c->ev[2], s));
     if (d_nhits) {
-        hipLaunchKernelGGL(store_count_kernel, dim3(1), dim3(64), 0, s, c->counters.p, d_nhits);
+        hipLaunchKernelGGL(store_count_kernel, dim3(1), dim3(64), 0, s, counters, d_nhits);
         HIP_TRY(hipGetLastError());
     }
     c->pending = true;
     c->pending_hit_cap = cap;
+    c->pending_cand_cap = cand_cap;
+    c->pending_counters = counters;
     c->pending_stream = s;
     return BLAMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t packed_words,
+                          const uint64_t* d_lcmask, uint64_t mask_words, uint64_t nbases,
+                          const uint64_t* d_run_starts, uint64_t nruns, blamm_hit* d_hits,
+                          uint64_t cap, uint64_t* d_nhits, void* stream)
+{
+    BLAMM_TRY(check_scan_args(c, d_packed, packed_words, d_lcmask, mask_words, nbases, d_run_starts, nruns, d_hits,
+                              cap));
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    if (c->cand.n == 0) HIP_TRY(c->cand.reserve(1u << 20));
+    return enqueue_scan(c, d_packed, d_lcmask, nbases, d_run_starts, nruns, d_hits, cap, d_nhits, c->cand.p,
+                        c->cand.n, c->counters.p, s);
 }
 
 int blamm_hip_scan_finish(blamm_hip_ctx* c, uint64_t* nhits)
@@ -731,7 +769,7 @@ int blamm_hip_scan_finish(blamm_hip_ctx* c, uint64_t* nhits)
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->pending_stream));
     unsigned long long cnt[kCntSlots];
-    HIP_TRY(hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cnt, c->pending_counters, sizeof(cnt), hipMemcpyDeviceToHost));
     float ms_f = 0.f, ms_all = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms_f, c->ev[0], c->ev[1]));
     HIP_TRY(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]));
@@ -741,10 +779,12 @@ int blamm_hip_scan_finish(blamm_hip_ctx* c, uint64_t* nhits)
     c->stats.kernel_ms = ms_all;
     c->stats.filter_ms = ms_f;
     if (nhits) *nhits = cnt[kCntHits];
-    if (cnt[kCntCand] > c->cand.n) {
+    if (cnt[kCntCand] > c->pending_cand_cap) {
+        // Filter records were dropped, so the hit count is incomplete: report
+        // 0 hits and the overflow.
+        if (nhits) *nhits = 0;
         // Grow the scratch so that a retry holds every candidate.
         HIP_TRY(c->cand.reserve((size_t)(cnt[kCntCand] + cnt[kCntCand] / 4 + 1024)));
-        if (nhits) *nhits = 0;
         return fail(BLAMM_ERR_OVERFLOW, "scan: candidate scratch overflowed (grown; retry the scan)");
     }
     if (cnt[kCntHits] > c->pending_hit_cap)

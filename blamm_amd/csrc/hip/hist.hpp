@@ -83,10 +83,19 @@ void hist_slot_params(const blamm_hip_ctx* c, const float* mins, const float* ma
     }
 }
 
-template <bool MASK, bool LDS>
+template <bool MASK, bool LDS, bool WIDE>
 const void* hist_kernel_ptr()
 {
-    return (const void*)hist_filter_kernel<MASK, LDS>;
+    return (const void*)hist_filter_kernel<MASK, LDS, WIDE>;
+}
+
+const void* hist_kernel(bool mask, bool lds, bool wide)
+{
+    if (wide)
+        return mask ? (lds ? hist_kernel_ptr<true, true, true>() : hist_kernel_ptr<true, false, true>())
+                    : (lds ? hist_kernel_ptr<false, true, true>() : hist_kernel_ptr<false, false, true>());
+    return mask ? (lds ? hist_kernel_ptr<true, true, false>() : hist_kernel_ptr<true, false, false>())
+                : (lds ? hist_kernel_ptr<false, true, false>() : hist_kernel_ptr<false, false, false>());
 }
 
 }  // namespace
@@ -124,9 +133,16 @@ int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     // fragments in registers at four waves per SIMD); its histograms live in
     // LDS when its 32 columns x nbins counters fit
     const bool lds = (uint64_t)nbins * kBf16Cols * 4 <= kHistLdsBytes;
+    // blocks of motifs up to 32 positions first, then the wide ones (their
+    // own launch at two waves per SIMD)
     std::vector<uint32_t> groups;
+    for (int pass = 0; pass < 2; ++pass)
+        for (const GroupInfo& g : c->groups)
+            if ((g.nch > (uint32_t)kWideChunks) == (pass == 1))
+                for (uint32_t i = 0; i < g.nb; ++i) groups.push_back(group_pack(g.blk0 + i, g.nch, 1));
+    uint32_t nnarrow = 0;
     for (const GroupInfo& g : c->groups)
-        for (uint32_t i = 0; i < g.nb; ++i) groups.push_back(group_pack(g.blk0 + i, g.nch, 1));
+        if (g.nch <= (uint32_t)kWideChunks) nnarrow += g.nb;
     HIP_TRY(c->hist_slots.reserve(hs.size()));
     HIP_TRY(c->hist_groups.reserve(groups.size()));
     const size_t ncount = (size_t)c->nslots * nbins;
@@ -136,12 +152,6 @@ int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
                            s));
     const bool mask = d_lcmask != nullptr;
     const uint32_t dyn = lds ? kBf16Cols * nbins * 4 : 0u;
-    const void* kp = mask ? (lds ? hist_kernel_ptr<true, true>() : hist_kernel_ptr<true, false>())
-                          : (lds ? hist_kernel_ptr<false, true>() : hist_kernel_ptr<false, false>());
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, kThreads, dyn) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-    const uint32_t grid = (uint32_t)c->num_cu * (uint32_t)per_cu;
 
     HistArgs a{};
     a.packed = d_packed;
@@ -151,7 +161,6 @@ int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     a.hs = c->hist_slots.p;
     a.slot_w = c->slot_w.p;
     a.run_starts = d_run_starts;
-    a.groups = c->hist_groups.p;
     a.counts = c->hist_counts.p;
     a.counters = c->counters.p;
     a.nbases = (uint32_t)nbases;
@@ -159,28 +168,37 @@ int blamm_hip_hist_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     a.qend = (uint32_t)qend;
     a.nbins = nbins;
     a.ntiles = (uint32_t)((qend + kBf16Tile - 1) / kBf16Tile);   // windows start before qend
-    a.ngroups = (uint32_t)groups.size();
-    {
-        // about four items per workgroup, 16..1024 tiles per item
-        const uint64_t want = ((uint64_t)a.ntiles * a.ngroups + 4ull * grid - 1) / (4ull * grid);
-        a.chunk_tiles = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(16, want));
-        const uint64_t nchunks = (a.ntiles + a.chunk_tiles - 1) / a.chunk_tiles;
-        if (nchunks * a.ngroups >= (1ull << 32)) return fail(BLAMM_ERR_ARG, "hist: work list too long");
-        a.nitems = (uint32_t)(nchunks * a.ngroups);
-    }
 
     HIP_TRY(hipMemsetAsync(c->hist_counts.p, 0, ncount * sizeof(unsigned long long), s));
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, c->counters.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[0], s));
-    if (a.nitems > 0) {
-        if (mask) {
-            if (lds) hipLaunchKernelGGL((hist_filter_kernel<true, true>), dim3(grid), dim3(kThreads), dyn, s, a);
-            else hipLaunchKernelGGL((hist_filter_kernel<true, false>), dim3(grid), dim3(kThreads), dyn, s, a);
+    for (int wide = 0; wide < 2; ++wide) {
+        HistArgs w = a;
+        w.groups = c->hist_groups.p + (wide ? nnarrow : 0);
+        w.ngroups = wide ? (uint32_t)groups.size() - nnarrow : nnarrow;
+        w.queue = (uint32_t)wide;
+        if (w.ngroups == 0 || w.ntiles == 0) continue;
+        const void* kp = hist_kernel(mask, lds, wide);
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, kThreads, dyn) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        const uint32_t grid = (uint32_t)c->num_cu * (uint32_t)per_cu;
+        // about four items per workgroup, 16..1024 tiles per item
+        const uint64_t want = ((uint64_t)w.ntiles * w.ngroups + 4ull * grid - 1) / (4ull * grid);
+        w.chunk_tiles = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(16, want));
+        const uint64_t nchunks = (w.ntiles + w.chunk_tiles - 1) / w.chunk_tiles;
+        if (nchunks * w.ngroups >= (1ull << 32)) return fail(BLAMM_ERR_ARG, "hist: work list too long");
+        w.nitems = (uint32_t)(nchunks * w.ngroups);
+#define BLAMM_HL(M, L, W) hipLaunchKernelGGL((hist_filter_kernel<M, L, W>), dim3(grid), dim3(kThreads), dyn, s, w)
+        if (wide) {
+            if (mask) { if (lds) BLAMM_HL(true, true, true); else BLAMM_HL(true, false, true); }
+            else { if (lds) BLAMM_HL(false, true, true); else BLAMM_HL(false, false, true); }
         } else {
-            if (lds) hipLaunchKernelGGL((hist_filter_kernel<false, true>), dim3(grid), dim3(kThreads), dyn, s, a);
-            else hipLaunchKernelGGL((hist_filter_kernel<false, false>), dim3(grid), dim3(kThreads), dyn, s, a);
+            if (mask) { if (lds) BLAMM_HL(true, true, false); else BLAMM_HL(true, false, false); }
+            else { if (lds) BLAMM_HL(false, true, false); else BLAMM_HL(false, false, false); }
         }
+#undef BLAMM_HL
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(c->ev[1], s));

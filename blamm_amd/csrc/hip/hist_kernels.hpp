@@ -48,6 +48,7 @@ struct HistArgs {
     unsigned long long* counters;  // kCntBand: windows binned exactly; kCntQueue: work item head
     uint32_t nbases, nruns, qend, nbins;
     uint32_t ntiles, chunk_tiles, ngroups, nitems;
+    uint32_t queue;                // work-item counter of this launch: counters[kCntQueue + queue]
 };
 
 // LDS histogram budget per workgroup (32 columns x up to 512 bins).
@@ -89,11 +90,12 @@ __device__ __forceinline__ uint32_t hist_exact_stage(const HistArgs& a, Stage& s
             rows &= rows - 1u;
             const uint32_t q = cd.x + 8 * (r >> 2) + (r & 3);
             const uint64_t bits = load_bits64(a.packed, 2ull * q);
+            const uint64_t bits2 = h.len > 32 ? load_bits64(a.packed, 2ull * q + 64) : 0ull;
             const uint64_t mbits = a.lcmask ? load_bits64(a.lcmask, q) : 0ull;
             float s = 0.f;
             for (uint32_t p = 0; p < h.len; ++p) {
                 const float4 wp = *(const float4*)(w + 4 * p);
-                const uint32_t b = (uint32_t)(bits >> (2 * p)) & 3u;
+                const uint32_t b = (uint32_t)(p < 32 ? bits >> (2 * p) : bits2 >> (2 * (p - 32))) & 3u;
                 const float x = b == 0 ? wp.x : b == 1 ? wp.y : b == 2 ? wp.z : wp.w;
                 if (!((mbits >> p) & 1)) s += x;
             }
@@ -203,8 +205,8 @@ __device__ __forceinline__ uint32_t hist_range(const HistArgs& a, Stage& st, uin
     }
     for (uint32_t t = t0; t < t1; t += kWavesPerBlock) {
         const uint32_t q0 = t * kBf16Tile;
-        const OneHotSrc bits = onehot_src<NCH>(pk[t], pk[t + 1], sh);
-        const uint32_t mb = MASK ? funnel32(mk[t], mk[t + 1], off & 31) : 0u;
+        const OneHotSrc bits = onehot_src<NCH>(pk[t], pk[t + 1], pk[t + 2], sh);
+        const uint64_t mb = MASK ? window_mask<NCH>(mk[t], mk[t + 1], mk[t + 2], off & 31) : 0ull;
         f32x16 acc[NB];
         bf16_mfma_tile<NCH, NB, MASK>(acc, bh, bl, bits, mb);
         while (rnext <= q0) {
@@ -241,8 +243,11 @@ __device__ __forceinline__ uint32_t hist_range(const HistArgs& a, Stage& st, uin
 // its LDS histogram.  Up to 128 VGPRs so that four waves per SIMD hide the
 // latency of the per-window binning (measured: 8.3 ms against 12.3 ms with
 // two blocks per wave at two waves per SIMD, 10 Mbp x 1,000 columns).
-template <bool MASK, bool LDS>
-__global__ __launch_bounds__(kThreads, 4) void hist_filter_kernel(HistArgs a)
+// Blocks of motifs longer than 32 positions (WIDE: up to 16 chunks of hi and
+// lo fragments in registers) run in a launch of their own at two waves per
+// SIMD.
+template <bool MASK, bool LDS, bool WIDE>
+__global__ __launch_bounds__(kThreads, WIDE ? 2 : 4) void hist_filter_kernel(HistArgs a)
 {
     extern __shared__ uint32_t hist_lds[];
     __shared__ uint4 stage_lds[kWavesPerBlock][kHistStage];
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(kThreads, 4) void hist_filter_kernel(HistArgs a)
     Stage st{stage_lds[wave], 0u, nullptr};
     uint32_t exact = 0;
     for (;;) {
-        if (threadIdx.x == 0) item_s = (uint32_t)atomicAdd(&a.counters[kCntQueue], 1ull);
+        if (threadIdx.x == 0) item_s = (uint32_t)atomicAdd(&a.counters[kCntQueue + a.queue], 1ull);
         __syncthreads();
         const uint32_t it = item_s;
         if (it >= a.nitems) break;
@@ -263,14 +268,22 @@ __global__ __launch_bounds__(kThreads, 4) void hist_filter_kernel(HistArgs a)
         if (LDS)
             for (uint32_t i = threadIdx.x; i < used; i += kThreads) hist_lds[i] = 0u;
         __syncthreads();
-        switch (nch * 8 + nb) {
 #define BLAMM_H(N, B) \
     case N * 8 + B: exact += hist_range<N, B, MASK, LDS>(a, st, hist_lds, blk0, t0 + wave, t1, lane); break;
-            BLAMM_H(1, 1) BLAMM_H(2, 1) BLAMM_H(3, 1) BLAMM_H(4, 1)
-            BLAMM_H(5, 1) BLAMM_H(6, 1) BLAMM_H(7, 1) BLAMM_H(8, 1)
-#undef BLAMM_H
-            default: break;
+        if constexpr (WIDE) {
+            switch (nch * 8 + nb) {
+                BLAMM_H(9, 1) BLAMM_H(10, 1) BLAMM_H(11, 1) BLAMM_H(12, 1)
+                BLAMM_H(13, 1) BLAMM_H(14, 1) BLAMM_H(15, 1) BLAMM_H(16, 1)
+                default: break;
+            }
+        } else {
+            switch (nch * 8 + nb) {
+                BLAMM_H(1, 1) BLAMM_H(2, 1) BLAMM_H(3, 1) BLAMM_H(4, 1)
+                BLAMM_H(5, 1) BLAMM_H(6, 1) BLAMM_H(7, 1) BLAMM_H(8, 1)
+                default: break;
+            }
         }
+#undef BLAMM_H
         if (LDS) {
             __syncthreads();
             unsigned long long* dst = a.counts + (size_t)blk0 * kBf16Cols * a.nbins;

@@ -27,8 +27,10 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWaveSize = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kThreads = kWaveSize * kWavesPerBlock;
-constexpr int kMaxLen = 32;          // positions per column slot
+constexpr int kMaxLen = 64;          // positions per column slot
 constexpr int kMaxChunks = kMaxLen / 4;
+constexpr int kWideChunks = 8;       // above this many chunks a wave holds one column block
+constexpr int kLoChunks = 16;        // lazy-lo form: chunk fragments per wave in LDS (NB x NCH)
 constexpr int kBf16Cols = 32;        // columns per bf16 column block (MFMA N)
 constexpr int kBf16Tile = 32;        // windows per bf16 tile (MFMA M)
 constexpr int kFp32Cols = 16;        // columns per fp32 column block
@@ -226,7 +228,7 @@ constexpr int kGroup = 4;
 // six bits of its count, so a chunk costs one 64-bit shift of p[h] (none for
 // chunks 0 and 4) and one 64-bit shift per position.
 struct OneHotSrc {
-    uint64_t p[2];
+    uint64_t p[4];
 };
 
 // Packed words as loaded by a lane: the lane's pointer is offset to the
@@ -234,30 +236,41 @@ struct OneHotSrc {
 typedef uint64_t u64_a4 __attribute__((aligned(4)));
 
 // Shift counts of the lane's window whose first base is at bit s (< 32) of
-// the dword pair lo; bases 16..31 of the window continue in the low dword of
-// hi.  Up to 4 chunks need only 32 bits.
-template <int NCH>
-__device__ __forceinline__ OneHotSrc onehot_src(uint64_t lo, uint64_t hi, uint32_t s)
+// the dword pair w0; the window's bases continue through w1 (bases 16..47)
+// and w2 (bases 48..63).  Every 4 chunks need 32 more bits.
+__device__ __forceinline__ uint64_t onehot_counts(uint32_t b)
 {
     constexpr uint32_t M = 0x30303030u;
-    OneHotSrc r;
-    const uint32_t b0 = funnel32((uint32_t)lo, (uint32_t)(lo >> 32), s);
-    r.p[0] = ((uint64_t)((b0 << 2) & M) << 32) | ((b0 << 4) & M);
+    uint64_t p = ((uint64_t)((b << 2) & M) << 32) | ((b << 4) & M);
     // an empty asm (no instruction) keeps the compiler from folding the
     // masks back into every per-chunk extraction
-    asm("" : "+v"(r.p[0]));
-    if constexpr (NCH > 4) {
-        const uint32_t b1 = funnel32((uint32_t)(lo >> 32), (uint32_t)hi, s);
-        r.p[1] = ((uint64_t)((b1 << 2) & M) << 32) | ((b1 << 4) & M);
-        asm("" : "+v"(r.p[1]));
-    } else {
-        r.p[1] = 0;
-    }
+    asm("" : "+v"(p));
+    return p;
+}
+
+template <int NCH>
+__device__ __forceinline__ OneHotSrc onehot_src(uint64_t w0, uint64_t w1, uint64_t w2, uint32_t s)
+{
+    OneHotSrc r;
+    r.p[0] = onehot_counts(funnel32((uint32_t)w0, (uint32_t)(w0 >> 32), s));
+    r.p[1] = NCH > 4 ? onehot_counts(funnel32((uint32_t)(w0 >> 32), (uint32_t)w1, s)) : 0;
+    r.p[2] = NCH > 8 ? onehot_counts(funnel32((uint32_t)w1, (uint32_t)(w1 >> 32), s)) : 0;
+    r.p[3] = NCH > 12 ? onehot_counts(funnel32((uint32_t)(w1 >> 32), (uint32_t)w2, s)) : 0;
     return r;
 }
 
+// Lowercase bits of the lane's window (bit p = position p), from consecutive
+// mask dwords m0, m1, m2 starting at bit s (< 32) of m0.
+template <int NCH>
+__device__ __forceinline__ uint64_t window_mask(uint32_t m0, uint32_t m1, uint32_t m2, uint32_t s)
+{
+    uint64_t m = funnel32(m0, m1, s);
+    if constexpr (NCH > kWideChunks) m |= (uint64_t)funnel32(m1, m2, s) << 32;
+    return m;
+}
+
 template <bool MASK>
-__device__ __forceinline__ bf16x8 onehot_chunk(const OneHotSrc& src, uint32_t mb, int j)
+__device__ __forceinline__ bf16x8 onehot_chunk(const OneHotSrc& src, uint64_t mb, int j)
 {
     uint64_t c = src.p[j >> 2];
     if (j & 3) {
@@ -280,7 +293,7 @@ __device__ __forceinline__ bf16x8 onehot_chunk(const OneHotSrc& src, uint32_t mb
 
 template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const bf16x8 (&bh)[NB][NCH],
-                                               const bf16x8 (&bl)[NB][NCH], const OneHotSrc& bits, uint32_t mb)
+                                               const bf16x8 (&bl)[NB][NCH], const OneHotSrc& bits, uint64_t mb)
 {
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
@@ -307,7 +320,7 @@ __device__ __forceinline__ void bf16_mfma_tile(f32x16 (&acc)[NB], const bf16x8 (
 // ---------------------------------------------------------------------------
 template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void bf16_hi_tile(f32x16 (&acc)[NB], const bf16x8 (&bh)[NB][NCH],
-                                             const OneHotSrc& bits, uint32_t mb)
+                                             const OneHotSrc& bits, uint64_t mb)
 {
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = (f32x16){0.f};
@@ -334,7 +347,7 @@ __device__ __forceinline__ float tile_max(const f32x16& v)
 template <int NCH, int NB, bool MASK, bool LO>
 __device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st, f32x16 (&acc)[NB],
                                                  const float (&th)[NB], const float (&thh)[NB], const OneHotSrc& bits,
-                                                 uint32_t mb, uint32_t q0, uint32_t blk0, uint32_t lane)
+                                                 uint64_t mb, uint32_t q0, uint32_t blk0, uint32_t lane)
 {
     uint32_t need = 0;
 #pragma unroll
@@ -385,7 +398,8 @@ template <int NCH, int NB, bool MASK, bool LO>
 __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint32_t blk0, uint32_t t0, uint32_t t1,
                                            uint32_t lane)
 {
-    static_assert(NB * NCH <= 2 * kMaxChunks, "lo fragments exceed the wave's LDS slice");
+    static_assert(!LO || NB * NCH <= kLoChunks, "lo fragments exceed the wave's LDS slice");
+    static_assert(NCH <= kWideChunks || NB == 1, "wide blocks are scored one per wave");
     const uint4* __restrict__ fh = (const uint4*)a.frag_a;
     const uint4* __restrict__ fl = (const uint4*)a.frag_b;
     bf16x8 bh[NB][NCH];
@@ -405,15 +419,17 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
 
     const uint32_t off = (lane & 31) + 2 * (lane >> 5);   // first base of the lane's window, 0..33
     // pk[t]: dwords 2t + off/16 and the next of the packed stream, so the
-    // window of tile t starts at bit sh < 32 of pk[t]
+    // window of tile t starts at bit sh < 32 of pk[t] and continues through
+    // pk[t + 1] (and pk[t + 2] for motifs longer than 48 positions)
     const uint32_t sh = 2 * (off & 15);
     const u64_a4* __restrict__ pk = (const u64_a4*)((const uint32_t*)a.packed + (off >> 4));
     const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask + (off >> 5);
     f32x16 accA[NB], accB[NB];
     OneHotSrc bA, bB{};
-    uint32_t mA = 0, mB = 0, m0 = 0, m1 = 0, m2 = 0;
-    bA = onehot_src<NCH>(pk[t0], pk[t0 + 1], sh);
-    if (MASK) mA = funnel32(mk[t0], mk[t0 + 1], off & 31);
+    uint64_t mA = 0, mB = 0;
+    uint32_t m0 = 0, m1 = 0, m2 = 0;
+    bA = onehot_src<NCH>(pk[t0], pk[t0 + 1], pk[t0 + 2], sh);
+    if (MASK) mA = window_mask<NCH>(mk[t0], mk[t0 + 1], mk[t0 + 2], off & 31);
     bf16_hi_tile<NCH, NB, MASK>(accA, bh, bA, mA);
     uint64_t w0 = pk[t0 + 1], w1 = pk[t0 + 2], w2 = pk[t0 + 3];
     if (MASK) { m0 = mk[t0 + 1]; m1 = mk[t0 + 2]; m2 = mk[t0 + 3]; }
@@ -423,20 +439,20 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         const uint64_t w3 = pk[t + 3], w4 = pk[t + 4];
         uint32_t m3 = 0, m4 = 0;
         if (MASK) { m3 = mk[t + 3]; m4 = mk[t + 4]; }
-        bB = onehot_src<NCH>(w0, w1, sh);
-        if (MASK) mB = funnel32(m0, m1, off & 31);
+        bB = onehot_src<NCH>(w0, w1, w2, sh);
+        if (MASK) mB = window_mask<NCH>(m0, m1, m2, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
         bf16_finish_tile<NCH, NB, MASK, LO>(a, st, accA, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
-        bA = onehot_src<NCH>(w1, w2, sh);
-        if (MASK) mA = funnel32(m1, m2, off & 31);
+        bA = onehot_src<NCH>(w1, w2, w3, sh);
+        if (MASK) mA = window_mask<NCH>(m1, m2, m3, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accA, bh, bA, mA);
         bf16_finish_tile<NCH, NB, MASK, LO>(a, st, accB, th, thh, bB, mB, t * kBf16Tile, blk0, lane);
         w0 = w2; w1 = w3; w2 = w4;
         if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
     if (t < t1) {                   // one tile left: t
-        bB = onehot_src<NCH>(w0, w1, sh);
-        if (MASK) mB = funnel32(m0, m1, off & 31);
+        bB = onehot_src<NCH>(w0, w1, w2, sh);
+        if (MASK) mB = window_mask<NCH>(m0, m1, m2, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
         bf16_finish_tile<NCH, NB, MASK, LO>(a, st, accA, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
         pendingA = false;
@@ -458,7 +474,7 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
 template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, const float (&bw)[NB][4 * NCH],
                                              const float (&th)[NB], uint32_t q0,
-                                             uint64_t bits, uint32_t mb, uint32_t k, uint32_t blk0,
+                                             uint64_t bits, uint64_t bits2, uint64_t mb, uint32_t k, uint32_t blk0,
                                              uint32_t lane)
 {
     constexpr int NP = 4 * NCH;
@@ -467,7 +483,7 @@ __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, con
     for (int b = 0; b < NB; ++b) acc[b] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        const uint32_t base = (uint32_t)(bits >> (2 * p)) & 3u;
+        const uint32_t base = (uint32_t)((p < 32 ? bits >> (2 * p) : bits2 >> (2 * (p - 32)))) & 3u;
         bool on = (base == k);
         if (MASK) on = on && !((mb >> p) & 1);
         const float av = on ? 1.0f : 0.0f;
@@ -484,13 +500,17 @@ __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, con
     }
 }
 
-// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1 (FP32 path).
+// Tiles [t0, t1) of column blocks blk0 .. blk0+NB-1 (FP32 path).  A register
+// window of kGroup + 2 packed words covers a group of tiles and the 64 bases
+// past its last window; the next group's words are loaded a group ahead.
 template <int NCH, int NB, bool MASK>
 __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint32_t blk0, uint32_t t0, uint32_t t1,
                                            uint32_t lane)
 {
+    static_assert(NCH <= kWideChunks || NB == 1, "wide blocks are scored one per wave");
     const float* __restrict__ fw = (const float*)a.frag_a;
     constexpr int NP = 4 * NCH;
+    constexpr bool WIDE = NCH > kWideChunks;
     float bw[NB][NP];
     float th[NB];
 #pragma unroll
@@ -504,10 +524,10 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
     const uint32_t s0 = 2 * i, s1 = 2 * (i + 16);    // bit offsets of the two sub-tiles
     const uint64_t* __restrict__ pk = a.packed;
     const uint32_t* __restrict__ mk = (const uint32_t*)a.lcmask;
-    uint64_t w[kGroup + 1];
-    uint32_t mw[kGroup + 1];
+    uint64_t w[kGroup + 2];
+    uint32_t mw[kGroup + 2];
 #pragma unroll
-    for (int u = 0; u <= kGroup; ++u) {
+    for (int u = 0; u <= kGroup + 1; ++u) {
         w[u] = pk[t0 + u];
         mw[u] = MASK ? mk[t0 + u] : 0u;
     }
@@ -517,8 +537,8 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
         if (t + kGroup < t1) {
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
-                nx[u] = pk[t + kGroup + 1 + u];
-                nm[u] = MASK ? mk[t + kGroup + 1 + u] : 0u;
+                nx[u] = pk[t + kGroup + 2 + u];
+                nm[u] = MASK ? mk[t + kGroup + 2 + u] : 0u;
             }
         }
 #pragma unroll
@@ -527,18 +547,30 @@ __device__ __forceinline__ void fp32_range(const FilterArgs& a, Stage& st, uint3
                 const uint32_t q0 = (t + u) * kBf16Tile;
                 const uint64_t b0 = funnel64(w[u], w[u + 1], s0, 63 - s0);
                 const uint64_t b1 = funnel64(w[u], w[u + 1], s1, 63 - s1);
-                const uint32_t m0 = MASK ? funnel32(mw[u], mw[u + 1], i) : 0u;
-                const uint32_t m1 = MASK ? funnel32(mw[u], mw[u + 1], i + 16) : 0u;
-                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0, b0, m0, k, blk0, lane);
-                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0 + kFp32Tile, b1, m1, k, blk0, lane);
+                const uint64_t c0 = WIDE ? funnel64(w[u + 1], w[u + 2], s0, 63 - s0) : 0ull;
+                const uint64_t c1 = WIDE ? funnel64(w[u + 1], w[u + 2], s1, 63 - s1) : 0ull;
+                uint64_t m0 = 0, m1 = 0;
+                if (MASK) {
+                    m0 = funnel32(mw[u], mw[u + 1], i);
+                    m1 = funnel32(mw[u], mw[u + 1], i + 16);
+                    if (WIDE) {
+                        m0 |= (uint64_t)funnel32(mw[u + 1], mw[u + 2], i) << 32;
+                        m1 |= (uint64_t)funnel32(mw[u + 1], mw[u + 2], i + 16) << 32;
+                    }
+                }
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0, b0, c0, m0, k, blk0, lane);
+                fp32_subtile<NCH, NB, MASK>(a, st, bw, th, q0 + kFp32Tile, b1, c1, m1, k, blk0, lane);
             }
         }
-        w[0] = w[kGroup];
-        mw[0] = mw[kGroup];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            w[u] = w[kGroup + u];
+            mw[u] = mw[kGroup + u];
+        }
 #pragma unroll
         for (int u = 0; u < kGroup; ++u) {
-            w[u + 1] = nx[u];
-            mw[u + 1] = nm[u];
+            w[u + 2] = nx[u];
+            mw[u + 2] = nm[u];
         }
     }
 }
@@ -597,6 +629,8 @@ __device__ __forceinline__ void filter_item(const FilterArgs& a, Stage& st, uint
         BLAMM_BF(5, 1) BLAMM_BF(6, 1) BLAMM_BF(7, 1) BLAMM_BF(8, 1)
         BLAMM_BF(1, 2) BLAMM_BF(2, 2) BLAMM_BF(3, 2) BLAMM_BF(4, 2)
         BLAMM_BF(5, 2) BLAMM_BF(6, 2) BLAMM_BF(7, 2) BLAMM_BF(8, 2)
+        BLAMM_BF(9, 1) BLAMM_BF(10, 1) BLAMM_BF(11, 1) BLAMM_BF(12, 1)
+        BLAMM_BF(13, 1) BLAMM_BF(14, 1) BLAMM_BF(15, 1) BLAMM_BF(16, 1)
 #undef BLAMM_BF
         default: break;
         }
@@ -609,6 +643,8 @@ __device__ __forceinline__ void filter_item(const FilterArgs& a, Stage& st, uint
         BLAMM_F32(5, 2) BLAMM_F32(6, 2) BLAMM_F32(7, 2) BLAMM_F32(8, 2)
         BLAMM_F32(1, 4) BLAMM_F32(2, 4) BLAMM_F32(3, 4) BLAMM_F32(4, 4)
         BLAMM_F32(5, 4) BLAMM_F32(6, 4) BLAMM_F32(7, 4) BLAMM_F32(8, 4)
+        BLAMM_F32(9, 1) BLAMM_F32(10, 1) BLAMM_F32(11, 1) BLAMM_F32(12, 1)
+        BLAMM_F32(13, 1) BLAMM_F32(14, 1) BLAMM_F32(15, 1) BLAMM_F32(16, 1)
 #undef BLAMM_F32
         default: break;
         }
@@ -622,9 +658,9 @@ __device__ __forceinline__ void filter_body(const FilterArgs& a, const WorkQueue
 {
     const uint32_t lane = threadIdx.x & 63;
     __shared__ uint4 stage_lds[kWavesPerBlock][kStage];
-    // lazy-lo form: per-wave lo fragments (2 blocks x 8 chunks x 64 lanes)
-    __shared__ uint4 lo_lds[LO ? kWavesPerBlock * 2 * kMaxChunks * 64 : 1];
-    Stage st{stage_lds[threadIdx.x >> 6], 0u, LO ? lo_lds + (threadIdx.x >> 6) * 2 * kMaxChunks * 64 : lo_lds};
+    // lazy-lo form: per-wave lo fragments (kLoChunks chunk fragments x 64 lanes)
+    __shared__ uint4 lo_lds[LO ? kWavesPerBlock * kLoChunks * 64 : 1];
+    Stage st{stage_lds[threadIdx.x >> 6], 0u, LO ? lo_lds + (threadIdx.x >> 6) * kLoChunks * 64 : lo_lds};
     const uint32_t home = blockIdx.x % kXcds;
     for (uint32_t hop = 0; hop < (uint32_t)kXcds; ++hop) {
         const uint32_t x = (home + hop) % kXcds;
@@ -671,9 +707,9 @@ __global__ __launch_bounds__(kThreads, BF ? 3 : 1) void filter_persistent_kernel
 }
 
 // The bf16 filter in its lazy-lo form (80 KiB of LDS per block: two blocks
-// per CU).
+// per CU, so two waves per SIMD and at most 256 registers).
 template <bool MASK, bool LIST>
-__global__ __launch_bounds__(kThreads) void filter_lo_kernel(FilterArgs a, WorkQueue q, ListArgs l)
+__global__ __launch_bounds__(kThreads, 2) void filter_lo_kernel(FilterArgs a, WorkQueue q, ListArgs l)
 {
     filter_body<true, MASK, LIST, true>(a, q, l);
 }
@@ -800,9 +836,12 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             const uint32_t r = act ? (uint32_t)__builtin_ctz(rows) : 0u;
             rows &= rows - 1u;
             const uint32_t q = q0 + record_offset<NR>(r);
-            uint64_t bits = 0, mbits = 0;
+            // bases 0..31 of the window in bits, 32..63 in bits2 (loaded for
+            // motifs longer than 32 positions only)
+            uint64_t bits = 0, bits2 = 0, mbits = 0;
             if (act) {
                 bits = load_bits64(a.packed, 2ull * q);
+                if (L > 32) bits2 = load_bits64(a.packed, 2ull * q + 64);
                 if (a.lcmask) mbits = load_bits64(a.lcmask, q);
             }
             // left-to-right FP32 sum of the window: each lane loads only the
@@ -811,9 +850,10 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             float s = 0.f;
             const int Lw = act ? L : 0;
             for (int p0 = 0; p0 < Lw; p0 += 4) {
+                const uint64_t src = p0 < 32 ? bits >> (2 * p0) : bits2 >> (2 * (p0 - 32));
                 float x[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) x[u] = w[4 * (p0 + u) + (uint32_t)((bits >> (2 * (p0 + u))) & 3u)];
+                for (int u = 0; u < 4; ++u) x[u] = w[4 * (p0 + u) + (uint32_t)((src >> (2 * u)) & 3u)];
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
                     if (p0 + u < Lw && !((mbits >> (p0 + u)) & 1)) s += x[u];
@@ -839,7 +879,8 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
                         double d = 0.0;
                         for (int p = 0; p < L; ++p) {
                             if ((mbits >> p) & 1) continue;
-                            d += (double)w[4 * p + ((bits >> (2 * p)) & 3)];
+                            const uint64_t src = p < 32 ? bits >> (2 * p) : bits2 >> (2 * (p - 32));
+                            d += (double)w[4 * p + (src & 3)];
                         }
                         atomicAdd(&a.counters_rw[kCntBand], 1ull);
                         if (d >= (double)thr) {
@@ -868,7 +909,7 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
     flush();
 }
 
-__global__ void zero_counters_kernel(unsigned long long* c)
+static __global__ void zero_counters_kernel(unsigned long long* c)
 {
     if (threadIdx.x < kCntSlots) c[threadIdx.x] = 0ull;
 }
@@ -895,7 +936,7 @@ struct GroupArgs {
     const uint32_t* req_start;            // [nreq] first batch position of each request
 };
 
-__global__ __launch_bounds__(kGroupThreads) void group_clear_kernel(GroupArgs g)
+static __global__ __launch_bounds__(kGroupThreads) void group_clear_kernel(GroupArgs g)
 {
     const uint32_t nreq = g.desc->nreq;
     for (uint32_t i = blockIdx.x * kGroupThreads + threadIdx.x; i < nreq; i += gridDim.x * kGroupThreads)
@@ -908,7 +949,7 @@ __device__ __forceinline__ uint64_t group_nhits(const GroupArgs& g)
     return n < g.cap ? n : g.cap;
 }
 
-__global__ __launch_bounds__(kGroupThreads) void group_count_kernel(GroupArgs g)
+static __global__ __launch_bounds__(kGroupThreads) void group_count_kernel(GroupArgs g)
 {
     __shared__ uint32_t cnt[kGroupLds];
     const uint32_t nreq = g.desc->nreq;
@@ -930,7 +971,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_count_kernel(GroupArgs g)
 }
 
 // One block: exclusive prefix over the request counts.
-__global__ __launch_bounds__(1024) void group_scan_kernel(GroupArgs g)
+static __global__ __launch_bounds__(1024) void group_scan_kernel(GroupArgs g)
 {
     __shared__ unsigned long long part[1024];
     const uint32_t nreq = g.desc->nreq;
@@ -957,7 +998,7 @@ __global__ __launch_bounds__(1024) void group_scan_kernel(GroupArgs g)
 
 // Each block scatters a contiguous range of hits: counts per request in LDS,
 // reserves one global range per request, then places hits by LDS atomics.
-__global__ __launch_bounds__(kGroupThreads) void group_scatter_kernel(GroupArgs g)
+static __global__ __launch_bounds__(kGroupThreads) void group_scatter_kernel(GroupArgs g)
 {
     __shared__ uint32_t cnt[kGroupLds];
     __shared__ unsigned long long base[kGroupLds];

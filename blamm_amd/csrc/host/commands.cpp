@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -23,8 +24,8 @@
 #include <thread>
 
 #include "../../../include/blamm_hip.h"
-#include "format_g6.hpp"
 #include "host.hpp"
+#include "occurrences.hpp"
 
 namespace blamm {
 
@@ -126,104 +127,6 @@ std::string dir_with_slash(std::string d)
     return d;
 }
 
-// Append an unsigned integer.
-inline void put_u64(std::string& s, uint64_t v)
-{
-    char b[24];
-    auto r = std::to_chars(b, b + sizeof(b), v);
-    s.append(b, r.ptr);
-}
-
-// Append a float as ostream's default formatting prints it (%g, 6 digits).
-inline void put_score(std::string& s, float f)
-{
-    char b[32];
-    s.append(b, put_g6(b, f));
-}
-
-struct ColumnInfo {
-    std::string name;
-    uint64_t len;
-    char strand;
-};
-
-// The occurrence file.  Each batch of formatted lines arrives as one string
-// per worker; the workers' strings are written at their final offsets by one
-// thread each (pwrite), so the copy into the page cache runs in parallel.  A
-// destination without offsets (a pipe) is written in order.
-class OutFile {
-public:
-    explicit OutFile(const std::string& path) : path_(path)
-    {
-        fd_ = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
-        if (fd_ < 0) throw std::runtime_error("Cannot write to file: " + path);
-        seekable_ = ::lseek(fd_, 0, SEEK_CUR) >= 0;
-    }
-    ~OutFile()
-    {
-        if (fd_ >= 0) ::close(fd_);
-    }
-    OutFile(const OutFile&) = delete;
-    OutFile& operator=(const OutFile&) = delete;
-
-    void write_parts(const std::vector<std::string>& parts)
-    {
-        if (!seekable_) {
-            for (const std::string& p : parts) put(p.data(), p.size(), -1);
-            return;
-        }
-        std::vector<uint64_t> at(parts.size());
-        for (size_t i = 0; i < parts.size(); ++i) {
-            at[i] = off_;
-            off_ += parts[i].size();
-        }
-        std::vector<std::string> err(parts.size());
-        auto one = [&](size_t i) {
-            try {
-                put(parts[i].data(), parts[i].size(), (int64_t)at[i]);
-            } catch (const std::exception& e) {
-                err[i] = e.what();
-            }
-        };
-        std::vector<std::thread> pool;
-        for (size_t i = 1; i < parts.size(); ++i) pool.emplace_back(one, i);
-        if (!parts.empty()) one(0);
-        for (auto& t : pool) t.join();
-        for (const std::string& e : err)
-            if (!e.empty()) throw std::runtime_error(e);
-    }
-
-    void close()
-    {
-        if (fd_ >= 0 && ::close(fd_) != 0) {
-            fd_ = -1;
-            throw std::runtime_error("Cannot write to file: " + path_);
-        }
-        fd_ = -1;
-    }
-
-private:
-    // all of [p, p + n) at offset `at` (or at the current position when at < 0)
-    void put(const char* p, size_t n, int64_t at)
-    {
-        while (n > 0) {
-            const ssize_t w = at >= 0 ? ::pwrite(fd_, p, n, (off_t)at) : ::write(fd_, p, n);
-            if (w < 0) {
-                if (errno == EINTR) continue;
-                throw std::runtime_error("Cannot write to file: " + path_ + " (" + std::strerror(errno) + ")");
-            }
-            p += w;
-            n -= (size_t)w;
-            if (at >= 0) at += w;
-        }
-    }
-
-    std::string path_;
-    int fd_ = -1;
-    bool seekable_ = true;
-    uint64_t off_ = 0;
-};
-
 // Phase timings on stderr when BLAMM_TIMING is set (diagnostics only).
 class PhaseClock {
 public:
@@ -242,64 +145,6 @@ private:
     std::chrono::steady_clock::time_point t_;
 };
 
-// One output line per hit whose position is below `payload`: seqName  blamm
-// motif  pos  pos+L  score  strand  .  .  (pwmscan.cpp:88-95); formatted by
-// `threads` workers, written in hit order.
-uint64_t write_hits(OutFile& os, const blamm_hit* hits, size_t n, uint64_t payload, const PackedStream& s,
-                    uint64_t base, const std::vector<ColumnInfo>& cols, const Group& g, int threads)
-{
-    if (n == 0) return 0;
-    threads = (int)std::max<size_t>(1, std::min<size_t>((size_t)threads, n / 4096 + 1));
-    std::vector<std::string> out((size_t)threads);
-    std::vector<std::string> err((size_t)threads);
-    std::vector<uint64_t> written((size_t)threads, 0);
-    auto work = [&](int t) {
-        const size_t b = n * (size_t)t / (size_t)threads, e = n * (size_t)(t + 1) / (size_t)threads;
-        // a local string: the size fields of neighbouring strings in `out`
-        // would share cache lines between workers
-        std::string o;
-        o.reserve((e - b) * 64);
-        uint64_t kept = 0;
-        for (size_t i = b; i < e; ++i) {
-            const blamm_hit& h = hits[i];
-            if (h.pos >= payload) continue;   // starts in the halo: the next segment reports it
-            ++kept;
-            uint64_t seq, pos;
-            s.locate(base + h.pos, seq, pos);
-            if (seq >= g.seqNames.size()) {
-                err[(size_t)t] = "sequence index beyond the dictionary of group " + g.name +
-                                 " (is " + g.name + "'s .dict stale?)";
-                return;
-            }
-            const ColumnInfo& c = cols[h.col];
-            o += g.seqNames[seq];
-            o += "\tblamm\t";
-            o += c.name;
-            o += '\t';
-            put_u64(o, pos);
-            o += '\t';
-            put_u64(o, pos + c.len);
-            o += '\t';
-            put_score(o, h.score);
-            o += '\t';
-            o += c.strand;
-            o += "\t.\t.\n";
-        }
-        written[(size_t)t] = kept;
-        out[(size_t)t] = std::move(o);
-    };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < threads; ++t) pool.emplace_back(work, t);
-    work(0);
-    for (auto& th : pool) th.join();
-    for (const std::string& e : err)
-        if (!e.empty()) throw std::runtime_error(e);
-    os.write_parts(out);
-    uint64_t total = 0;
-    for (uint64_t w : written) total += w;
-    return total;
-}
-
 struct HipContext {
     blamm_hip_ctx* ctx = nullptr;
     ~HipContext()
@@ -307,6 +152,24 @@ struct HipContext {
         if (ctx) blamm_hip_destroy(ctx);
     }
 };
+
+// Bases per scan / histogram segment: 2^28, or BLAMM_SEGMENT_BASES rounded
+// down to a multiple of 64 (at least 64).  Segments are cut at multiples of
+// 64 bases so that their packed words and lowercase words start on word
+// boundaries; each carries a halo of lmax - 1 bases.
+uint64_t segment_bases()
+{
+    uint64_t seg = 1ull << 28;
+    if (const char* e = std::getenv("BLAMM_SEGMENT_BASES")) {
+        const unsigned long long v = std::strtoull(e, nullptr, 10);
+        if (v > 0) seg = std::max<uint64_t>(64, (uint64_t)v & ~63ull);
+    }
+    return std::min<uint64_t>(seg, 1ull << 30);
+}
+
+// Hits one segment may leave in the engine's pinned buffer before the next
+// segment is cut shorter (64M hits = 1 GiB).
+constexpr uint64_t kSegmentHits = 1ull << 26;
 
 void hip_check(int rc, const char* what)
 {
@@ -365,9 +228,9 @@ static void empirical_histograms(HipContext& hc, int device, const Group& g, con
                                     BLAMM_PREC_BF16X2),
               "library upload");
     std::vector<uint64_t> counts(ncols * numBins, 0);
-    // segments of 2^30 bases with a halo of lmax - 1; windows are counted in
-    // the segment where they start
-    const uint64_t seg = 1ull << 30, halo = lmax - 1;
+    // segments with a halo of lmax - 1; windows are counted in the segment
+    // where they start
+    const uint64_t seg = segment_bases(), halo = lmax - 1;
     for (uint64_t s0 = 0; s0 < s.n; s0 += seg) {
         const uint64_t payload = std::min(seg, s.n - s0);
         const uint64_t len = std::min(s.n - s0, payload + halo);
@@ -620,12 +483,21 @@ int scan_main(int argc, char** argv)
         clock.tick("read and pack FASTA");
         open_device();
         clock.tick("device setup");
-        hip_check(blamm_hip_set_library(hc.ctx, P.data(), (int)ldp, (int)ncols, lengths.data(), thr.data(), precision),
+        // A NaN threshold (getScoreCutoff's 0/0 on an empty top bin, e.g. -pt 0)
+        // passes no window in the reference (R >= NaN is false, kernel.cu:29;
+        // pwmscan.cpp:117-119); +inf does the same on the device.
+        std::vector<float> thr_dev(thr);
+        for (float& t : thr_dev)
+            if (std::isnan(t)) t = INFINITY;
+        hip_check(blamm_hip_set_library(hc.ctx, P.data(), (int)ldp, (int)ncols, lengths.data(), thr_dev.data(),
+                                        precision),
                   "library upload");
         clock.tick("library upload");
         // Segments of the stream (multiples of 64 bases) with a halo of
         // lmax - 1 bases; hits starting in a halo belong to the next segment.
-        const uint64_t seg = 1ull << 30;
+        // A segment that leaves more than kSegmentHits hits makes the next
+        // ones proportionally shorter, which bounds the pinned hit buffer.
+        uint64_t seg = segment_bases();
         const uint64_t halo = lmax - 1;
         if (!tuneCache.empty() && s.n > 0) {
             // tile shapes from a sample of this group (the first 16 Mbp)
@@ -647,8 +519,7 @@ int scan_main(int argc, char** argv)
             std::cout << "GPU tile shapes: " << t.chunk_tiles << " tiles per item, " << t.blocks_per_wave
                       << " column blocks per wave, K padded to " << t.kpad_positions << " positions" << std::endl;
         }
-        for (uint64_t s0 = 0; s0 < s.n || (s0 == 0 && s.n == 0); s0 += seg) {
-            if (s.n == 0) break;
+        for (uint64_t s0 = 0; s0 < s.n;) {
             const uint64_t payload = std::min(seg, s.n - s0);
             const uint64_t len = std::min(s.n - s0, payload + halo);
             std::vector<uint64_t> runs;
@@ -669,6 +540,9 @@ int scan_main(int argc, char** argv)
             clock.tick("scan (upload, kernels, hit read-back)");
             total += write_hits(os, hits, nh, payload, s, s0, cols, g, (int)numThreads);
             clock.tick("format and write occurrences");
+            s0 += payload;
+            if (nh > kSegmentHits && payload == seg)
+                seg = std::max<uint64_t>(64, (uint64_t)((double)seg * (double)kSegmentHits / (double)nh) & ~63ull);
         }
     }
     os.close();

@@ -41,7 +41,7 @@ extern "C" {
 
 /* Longest motif the engine scores (positions); longer motifs are rejected by
  * blamm_hip_set_library with BLAMM_ERR_ARG. */
-#define BLAMM_MAX_MOTIF_LEN 32
+#define BLAMM_MAX_MOTIF_LEN 64
 
 enum blamm_status {
     BLAMM_OK = 0,
@@ -97,7 +97,8 @@ int blamm_hip_create(int device, blamm_hip_ctx **ctx);
 int blamm_hip_destroy(blamm_hip_ctx *ctx);
 
 /* Upload one group's pattern matrix and thresholds; they stay resident until
- * replaced.  Replaces cudaMalloc+cublasSetVector of P and of the per-column
+ * replaced (BLAMM_ERR_STATE while a scan_device is in flight).  Replaces
+ * cudaMalloc+cublasSetVector of P and of the per-column
  * thresholds (pwmscan.cpp:315-342).
  *   P:          column-major, ldp >= 4*max(lengths); P[c*ldp + 4*p + b] is the
  *               weight of base b (ACGT) at motif position p of column c
@@ -123,7 +124,8 @@ uint64_t blamm_hip_packed_words(uint64_t nbases);
 uint64_t blamm_hip_mask_words(uint64_t nbases);
 
 /* Reserve room for n filter candidates (device scratch).  Scans grow it on
- * demand; reserving up front keeps device allocation out of later calls. */
+ * demand; reserving up front keeps device allocation out of later calls.
+ * BLAMM_ERR_STATE while a scan_device is in flight. */
 int blamm_hip_reserve(blamm_hip_ctx *ctx, uint64_t n);
 
 /* Scan host buffers.  packed has blamm_hip_packed_words(nbases) words; lcmask
@@ -153,9 +155,13 @@ int blamm_hip_scan_device(blamm_hip_ctx *ctx, const uint64_t *d_packed, uint64_t
                           const uint64_t *d_run_starts, uint64_t nruns,
                           blamm_hit *d_hits, uint64_t cap, uint64_t *d_nhits, void *stream);
 
-/* Synchronise the last scan_device; *nhits = hits produced (or required
- * capacity on BLAMM_ERR_OVERFLOW, when the candidate scratch or the hit
- * buffer was too small -- scratch is grown, so a retry succeeds). */
+/* Synchronise the last scan_device.  On success *nhits = hits produced.
+ * BLAMM_ERR_OVERFLOW has two causes:
+ *   - the hit buffer was too small: *nhits = the required capacity;
+ *   - the filter records exceeded the candidate scratch: *nhits = 0 (the hit
+ *     count is incomplete); after scan_device the scratch has been grown, so
+ *     a retry succeeds (blamm_hip_last_stats().candidates holds the count).
+ * Allocates only to grow the candidate scratch of scan_device. */
 int blamm_hip_scan_finish(blamm_hip_ctx *ctx, uint64_t *nhits);
 
 /* Counters of the last finished scan. */

@@ -32,7 +32,7 @@ def scanner():
         yield sc
 
 
-def make_library(seed, nmotifs=30, lmin=1, lmax=32, pvalue=1e-3):
+def make_library(seed, nmotifs=30, lmin=1, lmax=64, pvalue=1e-3):
     text, runs, bg = stream_case(seed, 20_000)
     ms = synth.motifs(nmotifs, lmin, lmax, 0.5, seed=seed)
     pwms, thr = library(ms, bg, pvalue=pvalue)
@@ -181,7 +181,7 @@ def test_batch_buffers_grow(scanner, orc):
 @pytest.mark.parametrize("prec", PRECS)
 def test_tunings_agree_with_oracle(scanner, orc, prec):
     text, runs, bg = stream_case(91, 70_000, lower_every=29)
-    ms = synth.motifs(45, 1, 32, 0.5, seed=91)
+    ms = synth.motifs(45, 1, 64, 0.5, seed=91)
     pwms, thr = library(ms, bg, pvalue=1e-3)
     P, lengths, _ = ba.build_pattern_matrix(pwms)
     scanner.set_library(P, lengths, thr, prec)

@@ -51,6 +51,8 @@ def gpu_vs_oracle(sc, orc, text, runs, pwms, mn, mx, nbins, qend=None):
     (3, 7, 4, 16, 300),      # LDS histograms, one block per item
     (4, 0, 6, 24, 1000),     # histograms in HBM
     (5, 101, 1, 8, 2),
+    (6, 5, 30, 64, 250),     # motifs longer than 32 positions (their own launch)
+    (7, 0, 1, 64, 1000),
 ])
 def test_hist_matches_oracle(scanner, orc, seed, lower, lmin, lmax, nbins):
     say(f"hist seed={seed} lower/{lower} L={lmin}..{lmax} bins={nbins}")

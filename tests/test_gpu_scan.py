@@ -53,7 +53,8 @@ def check_against_oracle(sc, orc, text, runs, pwms, thr, prec, lower=None, packe
 
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("seed,lower,lmin,lmax,pval", [
-    (1, 0, 5, 30, 1e-3), (2, 13, 1, 32, 1e-3), (3, 0, 6, 20, 1e-4), (4, 101, 10, 32, 1e-2), (5, 7, 1, 8, 1e-3)])
+    (1, 0, 5, 30, 1e-3), (2, 13, 1, 32, 1e-3), (3, 0, 6, 20, 1e-4), (4, 101, 10, 32, 1e-2), (5, 7, 1, 8, 1e-3),
+    (6, 0, 33, 64, 1e-3), (7, 11, 20, 64, 1e-3)])
 def test_parity_random(scanner, orc, prec, seed, lower, lmin, lmax, pval):
     say(f"parity seed={seed} prec={prec} L={lmin}..{lmax} lower/{lower} p={pval}")
     text, runs, bg = stream_case(seed, 120_000, lower_every=lower)
@@ -135,9 +136,10 @@ def test_edge_cases(scanner, orc, prec):
 
 @pytest.mark.parametrize("prec", PRECS)
 def test_many_classes_and_partial_blocks(scanner, orc, prec):
-    """Columns of every chunk count, block sizes that are not multiples of 32."""
+    """Columns of every chunk count up to 64 positions, block sizes that are
+    not multiples of 32."""
     text, runs, bg = stream_case(51, 60_000, lower_every=0)
-    ms = synth.motifs(71, 1, 32, 0.5, seed=51)
+    ms = synth.motifs(140, 1, 64, 0.5, seed=51)
     pwms, thr = library(ms, bg, pvalue=1e-3)
     say(f"classes prec={prec} cols={len(pwms)}")
     check_against_oracle(scanner, orc, text, runs, pwms, thr, prec)
@@ -212,7 +214,7 @@ def test_hit_capacity_overflow_is_reported(scanner):
 
 
 def test_invalid_arguments(scanner):
-    w = [np.zeros((33, 4), np.float32)]
+    w = [np.zeros((ba.MAX_MOTIF_LEN + 1, 4), np.float32)]
     P, lengths, _ = ba.build_pattern_matrix(w)
     with pytest.raises(ba.BlammError) as e:
         scanner.set_library(P, lengths, np.zeros(1, np.float32))

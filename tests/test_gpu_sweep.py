@@ -2,7 +2,7 @@
 
 Each seed draws a stream length (including sizes below one tile and around
 tile and word boundaries), dense run breaks (runs shorter than the motifs
-included), lowercase density, motif lengths 1..32, thresholds from far below
+included), lowercase density, motif lengths 1..64, thresholds from far below
 to above every score and +inf (a skipped column), then checks the scan in both
 precisions and the empirical histograms count for count.
 """
@@ -44,8 +44,9 @@ def random_case(seed):
     runs = np.unique(np.concatenate([[0], rng.integers(0, max(n, 1), size=nbreak)])).astype(np.uint64)
     ncols = int(rng.integers(1, 90))
     pwms = []
+    lmax = 65 if seed % 3 == 0 else 33
     for _ in range(ncols):
-        L = int(rng.integers(1, 33))
+        L = int(rng.integers(1, lmax))
         kind = rng.integers(0, 3)
         if kind == 0:
             w = rng.normal(0, 1.5, size=(L, 4))
