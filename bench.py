@@ -1,16 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X `blamm scan` engine.
 
-Workload (the plain-GPU drop-in configuration): 1,000 motifs with lengths
-uniform in 5..30 (counts = round(100 Dirichlet(0.5))), forward + reverse
-complement = 2,000 PWM columns; one 100 Mbp record at GC 0.41 with the CG
-dinucleotide at 0.2x and 0.5% of positions in N-runs of 10..500; thresholds
-from the theoretical histogram at p = 1e-4.  Synthetic, seeded.
+Headline workload (the precision form): 2,000 motifs with lengths uniform in
+5..30 (counts = round(100 Dirichlet(0.3))), forward + reverse complement =
+4,000 PWM columns; 4 manifest groups of 62.5 Mbp each (GC uniform in
+0.35..0.50), every group with its own background, PWMs and thresholds (the
+theoretical histogram's cutoff at p = 1e-5).  Synthetic, seeded.
 
-A step is one scan of the whole resident stream against the whole resident
-library: MFMA filter kernels + exact finalize (run boundaries, FP64 band
-rescoring) + hit count read-back.  Inputs are in HBM before timing starts.
-Prints one JSON line.
+A step is one scan of all four groups, each against its own resident library
+(one engine context per group): MFMA filter kernel + exact finalize (run
+boundaries, FP64 band rescoring) + hit count read-back per group.  Inputs are
+in HBM before timing starts.  `value` is the split-BF16 (hi-only filter)
+throughput in Gbp*motifs/s; the FP32 scan of the same groups, the lazy-lo
+form and an FP64-oracle slice check ride in the same line, with the plain-GPU
+workload (100 Mbp x 2,000 columns), the batched and serving workloads, hist -e
+and the reference's CPU configuration as side legs.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -34,17 +38,35 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per filter launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (None when absent)."""
-    path = os.path.join(REPO, "profiles", "filter_persistent_config1_traffic.txt")
+SOURCE_FILES = ("blamm_amd/csrc/hip/scan_kernels.hpp", "blamm_amd/csrc/hip/blamm_hip.hip")
+
+
+def source_hash() -> str:
+    """SHA-256 (first 16 hex digits) of the kernel sources the profiles were
+    taken on; a profile whose stamp differs describes another build."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in SOURCE_FILES:
+        with open(os.path.join(REPO, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def profile_figures(name: str):
+    """`key: value` lines of a committed rocprofv3 summary under profiles/, or
+    None when it is absent or was taken on other kernel sources."""
+    path = os.path.join(REPO, "profiles", name)
+    out = {}
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
+            if ":" in line and not line.startswith(" "):
+                k, v = line.split(":", 1)
+                out[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    if out.get("source_sha") != source_hash():
+        return {"stale": True, "source_sha": out.get("source_sha")}
+    return out
 
 
 def library_for(ms, bg, pvalues):
@@ -266,7 +288,7 @@ def batched_leg(args):
         seg = codes[s0:s0 + int(lengths[r])]
         host[woff[r]:woff[r + 1]] = ba.pack_codes(seg)
     d_all = torch.from_numpy(host.view(np.int64)).to("cuda:0")
-    cap = 1 << 20
+    cap = int(nh_batch) + 4096
     d_hits = torch.empty(cap * 2, dtype=torch.int64, device="cuda:0")
     torch.cuda.synchronize()
     lib, ctx = sc.lib, sc.ctx
@@ -276,21 +298,32 @@ def batched_leg(args):
     cnt = ctypes.c_uint64()
 
     def per_sequence_pass():
-        tot = 0
+        """Each launch appends its hits after the previous launch's."""
+        at, counts = 0, []
         for r in range(len(lengths)):
-            rc = lib.blamm_hip_scan_device(ctx, base + 8 * int(woff[r]), total_words - int(woff[r]), None, 0,
-                                           int(lengths[r]), None, 0, d_hits.data_ptr(), cap, None, None)
-            if rc != 0:
-                raise RuntimeError(lib.blamm_hip_last_error().decode())
-            if lib.blamm_hip_scan_finish(ctx, ctypes.byref(cnt)) != 0:
-                raise RuntimeError(lib.blamm_hip_last_error().decode())
-            tot += cnt.value
-        return tot
+            for attempt in range(2):
+                rc = lib.blamm_hip_scan_device(ctx, base + 8 * int(woff[r]), total_words - int(woff[r]), None, 0,
+                                               int(lengths[r]), None, 0, d_hits.data_ptr() + 16 * at, cap - at,
+                                               None, None)
+                if rc != 0:
+                    raise RuntimeError(lib.blamm_hip_last_error().decode())
+                rc = lib.blamm_hip_scan_finish(ctx, ctypes.byref(cnt))
+                if rc == ba.BLAMM_ERR_OVERFLOW and cnt.value == 0 and attempt == 0:
+                    continue        # candidate scratch grown: run again
+                if rc != 0:
+                    raise RuntimeError(lib.blamm_hip_last_error().decode())
+                break
+            counts.append(int(cnt.value))
+            at += int(cnt.value)
+        return at, counts
 
     per_sequence_pass()
     t1 = time.perf_counter()
-    nh_seq = per_sequence_pass()
+    nh_seq, counts = per_sequence_pass()
     wall_seq = time.perf_counter() - t1
+    sh = d_hits[: 2 * nh_seq].cpu().numpy().view(ba.HIT_DTYPE).copy()
+    sh["pos"] = sh["pos"] + np.repeat(starts.astype(np.uint64), counts)
+    same = bool(len(bh) == len(sh) and np.array_equal(hit_keys(bh, 24), hit_keys(sh, 24)))
     sc.close()
     work = float(n) * ncols
     out = {
@@ -305,27 +338,22 @@ def batched_leg(args):
         "per_sequence_value": work / wall_seq / 1e9,
         "speedup_vs_per_sequence": wall_seq / (np.mean(dev) / 1e3),
         "hits": int(nh_batch), "hits_per_sequence_launches": int(nh_seq),
-        "same_hit_count": bool(nh_batch == nh_seq),
+        "identical_hit_set": same,
         "tuning": tuning,
     }
     del bh
     return out
 
 
-def precision_leg(args, orc_slice_bases=500_000):
-    """Split-BF16 vs FP32 on 4 manifest groups (own background, PWMs and
-    thresholds each), p = 1e-5; identical hit sets, and the max-abs score error
-    against the FP64 oracle on a slice."""
-    import torch
-    import blamm_amd as ba
+def precision_workload(args):
+    """The four groups of the precision form: per group its PWM columns,
+    p = 1e-5 thresholds, pattern matrix, packed stream and device copies."""
     from blamm_amd import synth
+    import blamm_amd as ba
     t0 = time.time()
     ms, recs = synth.config2(seed=2, n_each=args.precision_bases)
-    log(f"precision workload: {len(recs)} groups x {args.precision_bases} bases ({time.time() - t0:.1f}s)")
-    res = {"bf16_ms": 0.0, "bf16lo_ms": 0.0, "fp32_ms": 0.0, "groups": [], "identical_hit_sets": True}
-    total_work = 0.0
-    slice_check = None
-    for gi, (name, codes, gc) in enumerate(recs):
+    groups = []
+    for name, codes, gc in recs:
         bg = synth.counts_of(codes)
         pwms, thr = library_for(ms, bg, [1e-5])
         thr = thr[1e-5]
@@ -334,59 +362,86 @@ def precision_leg(args, orc_slice_bases=500_000):
         words = ba.pack_codes(codes)
         runs = np.zeros(1, np.uint64)
         d_codes, d_runs = to_device(words, n, runs)
-        g = {"group": name, "gc": gc, "bases": n}
-        keys = {}
-        for prec, tag in ((ba.PREC_BF16X2, "bf16"), (ba.PREC_BF16X2_LO, "bf16lo"), (ba.PREC_FP32, "fp32")):
-            ds = DeviceScan(P, L, thr, prec)
-            nh = ds.step(d_codes, n, d_runs, 1)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            for _ in range(max(1, args.steps // 3)):
-                nh = ds.step(d_codes, n, d_runs, 1)
-            dt = (time.perf_counter() - t1) / max(1, args.steps // 3)
-            res[f"{tag}_ms"] += dt * 1e3
-            g[f"{tag}_ms"] = dt * 1e3
-            g["hits"] = int(nh)
-            keys[tag] = hit_keys(ds.hits(nh))
-            if tag == "bf16":
-                g["score_err_bound"] = ds.sc.bounds()[1]
-            ds.close()
-        g["identical"] = bool(np.array_equal(keys["bf16"], keys["fp32"]) and np.array_equal(keys["bf16"], keys["bf16lo"]))
-        res["identical_hit_sets"] &= g["identical"]
-        total_work += float(n) * len(pwms)
-        res["groups"].append(g)
-        if gi == 0 and orc_slice_bases:
-            # FP64 oracle on a slice of the first group
-            sys.path.insert(0, os.path.join(REPO, "oracle"))
-            import oracle
-            m = min(n, orc_slice_bases)
-            text = np.frombuffer(b"ACGT", np.uint8)[codes[:m]].tobytes()
-            ref = oracle.default().scan_fp64(text, [0], pwms, thr)
-            sc = ba.HipScanner(0)
-            sc.set_library(P, L, thr, ba.PREC_BF16X2)
-            got = sc.scan(ba.pack_codes(codes[:m]), m, [0], None)
-            sc.close()
-            a = np.lexsort((got["col"], got["pos"]))
-            bb = np.lexsort((ref["col"], ref["pos"]))
-            same = bool(len(got) == len(ref) and np.array_equal(got["pos"][a], ref["pos"][bb])
-                        and np.array_equal(got["col"][a], ref["col"][bb]))
-            err = float(np.max(np.abs(got["score"][a].astype(np.float64) - ref["score64"][bb]))) if same and len(got) else None
-            slice_check = {"bases": m, "hits": int(len(ref)), "hit_set_diff": 0 if same else
-                           int(len(set(zip(got["pos"], got["col"])) ^ set(zip(ref["pos"], ref["col"])))),
-                           "max_abs_score_err_vs_fp64": err}
-        del d_codes, d_runs
-    ncols = 2 * len(ms.pfms)
-    res.update({
-        "workload": f"precision config: {len(ms.pfms)} motifs (L 5-30, Dirichlet 0.3) fwd+rc = {ncols} columns, "
-                    f"{len(recs)} groups x {args.precision_bases // 1_000_000} Mbp GC U(0.35,0.50), p=1e-5",
-        "value": total_work / (res["bf16_ms"] / 1e3) / 1e9,
-        "fp32_value": total_work / (res["fp32_ms"] / 1e3) / 1e9,
-        "lazy_lo_value": total_work / (res["bf16lo_ms"] / 1e3) / 1e9,
-        "unit": "Gbp*motifs/s",
-        "speedup_vs_fp32": res["fp32_ms"] / res["bf16_ms"],
-        "fp64_slice": slice_check,
-    })
-    return res
+        groups.append({"name": name, "gc": gc, "codes": codes, "n": n, "pwms": pwms, "thr": thr, "P": P, "L": L,
+                       "d_codes": d_codes, "d_runs": d_runs})
+    log(f"precision workload: {len(groups)} groups x {args.precision_bases} bases, "
+        f"{len(groups[0]['pwms'])} columns ({time.time() - t0:.1f}s)")
+    return ms, groups
+
+
+def groups_leg(args, groups, precision, steps, warmup, tune_cache=None):
+    """One context per group with its library resident; `steps` timed passes
+    over all groups.  Returns timings and each group's hits."""
+    import torch
+    import blamm_amd as ba
+    ctxs = [DeviceScan(g["P"], g["L"], g["thr"], precision) for g in groups]
+    tuning = None
+    if tune_cache is not None:
+        g = groups[0]
+        tuning = ctxs[0].tune(g["d_codes"], g["n"], g["d_runs"], 0, tune_cache)
+        for c in ctxs[1:]:
+            c.sc.set_tuning(tuning["chunk_tiles"], tuning["blocks_per_wave"], tuning["kpad_positions"])
+
+    def one_pass():
+        res = []
+        for c, g in zip(ctxs, groups):
+            nh = c.step(g["d_codes"], g["n"], g["d_runs"], 0)
+            st = c.sc.stats()
+            res.append((nh, st.filter_ms, st.kernel_ms, int(st.candidates), int(st.band_rescored)))
+        return res
+
+    for _ in range(max(1, warmup)):
+        last = one_pass()
+    torch.cuda.synchronize()
+    filt, kern = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        last = one_pass()
+        filt += [r[1] for r in last]
+        kern += [r[2] for r in last]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    out = {"ms_per_step": dt * 1e3, "hits": int(sum(r[0] for r in last)),
+           "candidates": int(sum(r[3] for r in last)), "band_rescored": int(sum(r[4] for r in last)),
+           "filter_ms_per_launch": float(np.mean(filt)), "kernel_ms_per_group": float(np.mean(kern)),
+           "filter_band_max": max(c.sc.bounds()[0] for c in ctxs),
+           "score_err_bound": max(c.sc.bounds()[1] for c in ctxs),
+           "tuning": tuning or ctxs[0].sc.tuning().as_dict()}
+    hits = [c.hits(r[0]) for c, r in zip(ctxs, last)]
+    for c in ctxs:
+        c.close()
+    return out, hits
+
+
+def fp64_slices(groups, hits, slice_bases):
+    """The FP64 oracle on one slice of every group against the engine's hits
+    there: hit-set difference and the largest score error."""
+    import blamm_amd as ba
+    from blamm_amd import synth
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    orc = oracle.default()
+    rng = np.random.default_rng(7)
+    diff, worst, nref, bases = 0, 0.0, 0, 0
+    for g, h in zip(groups, hits):
+        n = g["n"]
+        a = int(rng.integers(0, max(1, n - slice_bases)))
+        b = min(n, a + slice_bases)
+        lmax = int(g["L"].max())
+        text = synth.CODE2BASE[g["codes"][a:b]].tobytes()
+        ref = orc.scan_fp64(text, [0], g["pwms"], g["thr"])
+        keep = ref["pos"] + lmax <= b - a
+        want = {(int(p) + a, int(c)): float(s) for p, c, s in zip(ref["pos"][keep], ref["col"][keep],
+                                                                 ref["score64"][keep])}
+        sel = (h["pos"] >= a) & (h["pos"] <= b - lmax)
+        got = {(int(p), int(c)): float(s) for p, c, s in zip(h["pos"][sel], h["col"][sel], h["score"][sel])}
+        diff += len(set(got) ^ set(want))
+        nref += len(want)
+        bases += b - a
+        for k in set(got) & set(want):
+            worst = max(worst, abs(got[k] - want[k]))
+    return {"bases": bases, "slices": len(groups), "hits": nref, "hit_set_diff": diff,
+            "max_abs_score_err_vs_fp64": worst}
 
 
 def serve_leg(args):
@@ -600,11 +655,38 @@ def cli_leg(args):
             "occurrence_bytes": size, "threads": os.cpu_count()}
 
 
-def cpu_leg(stream, runs, pwms, thr, budget_s: float):
-    """The reference's sgemm scan (oracle restatement, OpenBLAS, BLAS threads 1)."""
+def cpu_info():
+    """Host cores this process may use (its affinity set), the CPU share it
+    uses (OMP_NUM_THREADS when set, the box's share of cores for one GPU) and
+    the CPU model."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    share = affinity
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        share = min(share, int(env))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return affinity, share, model
+
+
+def cpu_leg(stream, runs, pwms, thr, budget_s: float, what: str):
+    """The reference's CPU path -- strip matrices of h = 1000 rows x w = 250
+    offsets, column tiles of the length-sorted pattern matrix, one OpenBLAS
+    sgemm per tile and offset (BLAS threads 1), one worker per core
+    (pwmscan.cpp:223-294) -- restated by the oracle, timed on a bounded prefix
+    of the stream."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
-    cores = min(16, os.cpu_count() or 1)
+    affinity, cores, model = cpu_info()
     text = np.frombuffer(b"ACGT", np.uint8)[stream]
     block = 1000 * 250
 
@@ -625,8 +707,63 @@ def cpu_leg(stream, runs, pwms, thr, budget_s: float):
     m, t = run(cores * block * rounds) if rounds > 1 else (m1, t1)
     value = m * len(pwms) / t / 1e9
     return {"value": value, "unit": "Gbp*motifs/s", "cores": cores, "kind": "port",
-            "sample": f"first {m} bases of the same stream x {len(pwms)} columns, sgemm path "
-                      f"(h=1000, w=250, OpenBLAS sgemm, BLAS threads 1), {t:.1f} s"}
+            "affinity_cores": affinity, "cpu_model": model,
+            "sample": f"first {m} bases of {what} x {len(pwms)} columns, sgemm path "
+                      f"(h=1000, w=250, OpenBLAS sgemm, BLAS threads 1, {cores} workers), {t:.1f} s"}
+
+
+def config0_leg(args):
+    """The reference's CPU configuration (64 motifs L 6-20 fwd+rc = 128
+    columns; 4 records of 250 kbp, 1% N-runs; p = 1e-4): the CPU path timed
+    whole on the host's share of cores, the GPU scan of the same stream, and
+    both hit sets against the FP64 oracle."""
+    import torch
+    import blamm_amd as ba
+    from blamm_amd import synth
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    ms, recs = synth.config0(seed=0)
+    stream, runs = synth.filtered(recs)
+    bg = synth.counts_of(stream)
+    pwms, thr = library_for(ms, bg, [1e-4])
+    thr = thr[1e-4]
+    n = int(stream.size)
+    text = np.frombuffer(b"ACGT", np.uint8)[stream].tobytes()
+    orc = oracle.default()
+    ref = orc.scan_fp64(text, runs, pwms, thr)
+    want = set(zip(ref["pos"].tolist(), ref["col"].tolist()))
+    affinity, cores, model = cpu_info()
+    order = sorted(range(len(pwms)), key=lambda c: pwms[c].shape[0])
+    reps, t_cpu = 0, 0.0
+    while t_cpu < 2.0 or reps < 3:
+        t0 = time.perf_counter()
+        cpu_hits = oracle.scan_sgemm(text, runs, [pwms[c] for c in order], thr[order], threads=cores)
+        t_cpu += time.perf_counter() - t0
+        reps += 1
+    t_cpu /= reps
+    cpu_set = {(int(p), int(order[c])) for p, c in zip(cpu_hits["pos"], cpu_hits["col"])}
+    P, L, _ = ba.build_pattern_matrix(pwms)
+    ds = DeviceScan(P, L, thr, ba.PREC_BF16X2)
+    d_codes, d_runs = to_device(ba.pack_codes(stream), n, runs)
+    for _ in range(3):
+        nh = ds.step(d_codes, n, d_runs, int(runs.size))
+    torch.cuda.synchronize()
+    k = 20
+    t0 = time.perf_counter()
+    for _ in range(k):
+        nh = ds.step(d_codes, n, d_runs, int(runs.size))
+    t_gpu = (time.perf_counter() - t0) / k
+    h = ds.hits(nh)
+    ds.close()
+    gpu_set = set(zip(h["pos"].tolist(), h["col"].tolist()))
+    work = float(n) * len(pwms)
+    return {"workload": f"reference CPU config: {len(pwms) // 2} motifs (L 6-20) fwd+rc = {len(pwms)} columns, "
+                        f"4 records x 250 kbp GC 0.41 1% N-runs, p=1e-4",
+            "bases": n, "columns": len(pwms), "hits_fp64": len(want),
+            "cpu": {"value": work / t_cpu / 1e9, "ms": t_cpu * 1e3, "cores": cores, "cpu_model": model,
+                    "kind": "port", "hit_set_diff_vs_fp64": len(cpu_set ^ want),
+                    "note": "the FP32 sgemm path; windows within FP32 rounding of a threshold may differ"},
+            "gpu": {"value": work / t_gpu / 1e9, "ms": t_gpu * 1e3, "hit_set_diff_vs_fp64": len(gpu_set ^ want)}}
 
 
 def main():
@@ -634,20 +771,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--bases", type=int, default=100_000_000)
-    ap.add_argument("--pvalue", type=float, default=1e-4)
+    ap.add_argument("--bases", type=int, default=100_000_000, help="bases of the plain-GPU workload")
+    ap.add_argument("--pvalue", type=float, default=1e-4, help="p-value of the plain-GPU workload")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-lo", action="store_true", help="skip the lazy-lo form of the split-BF16 scan")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64-oracle slices")
     ap.add_argument("--tune-cache", default=os.path.join(os.path.expanduser("~"), ".cache", "blamm_amd", "tune.txt"))
-    ap.add_argument("--sections", default="batched,precision,serve,hist",
-                    help="extra workloads after the headline scan: batched, precision, serve, hist, cli "
+    ap.add_argument("--sections", default="plain,batched,serve,hist,config0",
+                    help="side workloads after the headline: plain, batched, serve, hist, config0, cli "
                          "(comma list, or none; cli writes ~2 GB of occurrences to a temporary directory)")
     ap.add_argument("--batched-seqs", type=int, default=20_000)
     ap.add_argument("--precision-bases", type=int, default=62_500_000)
+    ap.add_argument("--slice-bases", type=int, default=150_000, help="bases per group of the FP64-oracle check")
     ap.add_argument("--serve-pool", type=int, default=32)
     ap.add_argument("--serve-max-seqs", type=int, default=5000)
     ap.add_argument("--serve-budget-gib", type=int, default=32)
@@ -674,36 +813,33 @@ def main():
     import blamm_amd as ba
     if args.tune_cache:
         os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
-    stream, runs, pwms, thr, P, lengths, codes = make_workload(args.seed, args.bases, args.pvalue)
-    nbp = int(stream.size)
-    ncols = len(pwms)
-    flops = float(nbp) * 8.0 * float(np.sum(lengths, dtype=np.float64))
-    log("split-bf16 leg")
-    bf, bf_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_BF16X2, tune_sample=nbp)
+    ms, groups = precision_workload(args)
+    ncols = len(groups[0]["pwms"])
+    nbp = sum(g["n"] for g in groups)
+    lsum = float(np.sum(groups[0]["L"], dtype=np.float64))
+    flops_step = float(nbp) * 8.0 * lsum
+    flops_launch = flops_step / len(groups)
+    tune = None if args.no_tune else args.tune_cache
+    log("split-bf16 (headline)")
+    bf, bf_hits = groups_leg(args, groups, ba.PREC_BF16X2, args.steps, args.warmup, tune)
     log(f"split-bf16: {bf}")
-    lo = None
-    if not args.no_lo:
-        log("split-bf16 lazy-lo leg")
-        lo, lo_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_BF16X2_LO, tune_sample=nbp)
-        log(f"split-bf16 lazy lo: {lo}")
-        a = np.sort(bf_hits["pos"].astype(np.uint64) * np.uint64(1 << 16) + bf_hits["col"])
-        b = np.sort(lo_hits["pos"].astype(np.uint64) * np.uint64(1 << 16) + lo_hits["col"])
-        lo["identical_hit_set"] = bool(a.size == b.size and np.array_equal(a, b))
-        del lo_hits
-    fp = None
-    if not args.no_fp32:
-        log("fp32 leg")
-        fp, fp_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_FP32, tune_sample=16_000_000)
-        log(f"fp32: {fp}")
-        a = np.sort(bf_hits["pos"].astype(np.uint64) * np.uint64(1 << 16) + bf_hits["col"])
-        b = np.sort(fp_hits["pos"].astype(np.uint64) * np.uint64(1 << 16) + fp_hits["col"])
-        fp["identical_hit_set"] = bool(a.size == b.size and np.array_equal(a, b))
+    keys_bf = [hit_keys(h) for h in bf_hits]
     value = nbp * ncols / (bf["ms_per_step"] / 1e3) / 1e9
-    traffic = measured_traffic()
-    achieved = flops / (bf["filter_ms"] / 1e3) / 1e12
-    alg_bytes = 0.25 * nbp + 16.0 * bf["hits"]
+    achieved = flops_launch / (bf["filter_ms_per_launch"] / 1e3) / 1e12
+    prof = profile_figures("filter_persistent_config2_traffic.txt") or {}
+    traffic = None
+    if prof and not prof.get("stale") and "traffic_bytes_per_launch" in prof:
+        traffic = float(prof["traffic_bytes_per_launch"])
+    pbytes = 4 * 4 * int(groups[0]["L"].max()) * ncols
+    alg_launch = 0.25 * nbp / len(groups) + 16.0 * bf["hits"] / len(groups) + pbytes
+    hbm = {"algorithmic_bytes_per_group": alg_launch,
+           "algorithmic_bytes_formula": "0.25 B/base (2-bit stream) + 16 B/hit + the FP32 pattern matrix once",
+           "achieved_GBs": alg_launch / (bf["kernel_ms_per_group"] / 1e3) / 1e9, "peak_GBs": PEAK_HBM_GBS}
+    if prof and not prof.get("stale") and "scan_traffic_bytes_per_group" in prof:
+        hbm["measured_scan_traffic_per_group"] = float(prof["scan_traffic_bytes_per_group"])
+        hbm["traffic_over_algorithmic"] = float(prof["scan_traffic_bytes_per_group"]) / alg_launch
     line = {
-        "metric": "Gbp*motifs/s (split-BF16 scan, FP64-exact hit set)",
+        "metric": "Gbp*motifs/s (split-BF16 scan vs FP32 GPU scan, FP64-exact hit set)",
         "value": value,
         "unit": "Gbp*motifs/s",
         "n_gpus": 1,
@@ -715,59 +851,101 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16x2 (hi terms on MFMA)->f32 (+f32/f64 rescoring)",
         "data": "synthetic (seeded)",
-        "config": {"workload": f"plain-GPU config: {ncols // 2} motifs (L 5-30) fwd+rc = {ncols} columns, "
-                               f"{args.bases // 1_000_000} Mbp GC 0.41 CpG 0.2x 0.5% N-runs, p={args.pvalue:g}",
-                   "bases": nbp, "columns": ncols, "pvalue": args.pvalue,
+        "config": {"workload": f"precision config: {ncols // 2} motifs (L 5-30, Dirichlet 0.3) fwd+rc = {ncols} "
+                               f"columns, {len(groups)} groups x {args.precision_bases / 1e6:g} Mbp GC U(0.35,0.50), "
+                               f"own background/PWMs/thresholds per group, p=1e-5",
+                   "bases": nbp, "columns": ncols, "groups": len(groups), "pvalue": 1e-5,
                    "precision": "split-bf16, hi-only filter"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
-                     "kernel": "filter_persistent_kernel<true,false,false> (one launch per step)",
-                     "algorithmic_flops": flops},
-        "hbm": {"algorithmic_bytes": alg_bytes, "achieved_GBs": alg_bytes / (bf["kernel_ms"] / 1e3) / 1e9,
-                "peak_GBs": PEAK_HBM_GBS},
+                     "kernel": "filter_persistent_kernel<true,false,false> (one launch per group)",
+                     "algorithmic_flops_per_launch": flops_launch,
+                     "traffic_source": "profiles/filter_persistent_config2_traffic.txt"
+                                       + ("" if traffic is not None else " (absent or taken on other sources)"),
+                     "source_sha": source_hash()},
+        "hbm": hbm,
         "scan": bf,
     }
-    if "host_api" in bf:
-        bf["host_api"]["value"] = nbp * ncols / (bf["host_api"]["ms_per_scan"] / 1e3) / 1e9
-        bf["host_api"]["value_pinned"] = nbp * ncols / (bf["host_api"]["ms_per_scan_pinned"] / 1e3) / 1e9
-    if lo is not None:
+    if not args.no_lo:
+        log("split-bf16 lazy-lo")
+        lo, lo_hits = groups_leg(args, groups, ba.PREC_BF16X2_LO, max(2, args.steps // 2), 1)
         lo["value"] = nbp * ncols / (lo["ms_per_step"] / 1e3) / 1e9
-        lach = flops / (lo["filter_ms"] / 1e3) / 1e12
-        lo.update({"roofline_achieved_TFLOPs": lach, "roofline_frac": lach / PEAK_BF16_TFLOPS})
+        lach = flops_launch / (lo["filter_ms_per_launch"] / 1e3) / 1e12
+        lo.update({"roofline_achieved_TFLOPs": lach, "roofline_frac": lach / PEAK_BF16_TFLOPS,
+                   "identical_hit_set": all(np.array_equal(a, hit_keys(b)) for a, b in zip(keys_bf, lo_hits))})
+        del lo_hits
         line["split_lazy_lo"] = lo
-    if fp is not None:
-        fval = nbp * ncols / (fp["ms_per_step"] / 1e3) / 1e9
-        fach = flops / (fp["filter_ms"] / 1e3) / 1e12
-        fp.update({"value": fval, "roofline_achieved_TFLOPs": fach, "roofline_frac": fach / PEAK_FP32_TFLOPS})
-        line["fp32"] = fp
-        line["speedup_vs_fp32"] = value / fval
-    del bf_hits
-    if fp is not None:
+        log(f"lazy lo: {lo}")
+    if not args.no_fp32:
+        log("fp32")
+        fp, fp_hits = groups_leg(args, groups, ba.PREC_FP32, 2, 1)
+        fp["value"] = nbp * ncols / (fp["ms_per_step"] / 1e3) / 1e9
+        fach = flops_launch / (fp["filter_ms_per_launch"] / 1e3) / 1e12
+        fp.update({"roofline_achieved_TFLOPs": fach, "roofline_frac": fach / PEAK_FP32_TFLOPS,
+                   "identical_hit_set": all(np.array_equal(a, hit_keys(b)) for a, b in zip(keys_bf, fp_hits))})
         del fp_hits
+        line["fp32"] = fp
+        line["speedup_vs_fp32"] = value / fp["value"]
+        log(f"fp32: {fp}")
+    if not args.no_check:
+        log("FP64 oracle slices")
+        line["fp64_slices"] = fp64_slices(groups, bf_hits, args.slice_bases)
+        log(f"fp64 slices: {line['fp64_slices']}")
+    del bf_hits, keys_bf
+    for g in groups:
+        del g["d_codes"], g["d_runs"]
     sections = [x for x in args.sections.split(",") if x and x != "none"]
+    if "plain" in sections:
+        log("plain-GPU workload")
+        stream, runs, pwms, thr, P, lengths, codes = make_workload(args.seed, args.bases, args.pvalue)
+        p_nbp = int(stream.size)
+        p_flops = float(p_nbp) * 8.0 * float(np.sum(lengths, dtype=np.float64))
+        pb, pb_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_BF16X2, tune_sample=p_nbp)
+        pb["value"] = p_nbp * len(pwms) / (pb["ms_per_step"] / 1e3) / 1e9
+        pach = p_flops / (pb["filter_ms"] / 1e3) / 1e12
+        pb.update({"roofline_achieved_TFLOPs": pach, "roofline_frac": pach / PEAK_BF16_TFLOPS,
+                   "workload": f"plain-GPU config: {len(pwms) // 2} motifs (L 5-30) fwd+rc = {len(pwms)} columns, "
+                               f"{args.bases // 1_000_000} Mbp GC 0.41 CpG 0.2x 0.5% N-runs, p={args.pvalue:g}",
+                   "bases": p_nbp, "columns": len(pwms)})
+        if "host_api" in pb:
+            pb["host_api"]["value"] = p_nbp * len(pwms) / (pb["host_api"]["ms_per_scan"] / 1e3) / 1e9
+            pb["host_api"]["value_pinned"] = p_nbp * len(pwms) / (pb["host_api"]["ms_per_scan_pinned"] / 1e3) / 1e9
+        if not args.no_fp32:
+            pf, pf_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_FP32, tune_sample=16_000_000)
+            pf["value"] = p_nbp * len(pwms) / (pf["ms_per_step"] / 1e3) / 1e9
+            pf["identical_hit_set"] = bool(np.array_equal(hit_keys(pb_hits), hit_keys(pf_hits)))
+            pb["fp32"] = pf
+            pb["speedup_vs_fp32"] = pb["value"] / pf["value"]
+            del pf_hits
+        del pb_hits
+        line["plain_gpu"] = pb
+        log(f"plain: {pb}")
+        if "hist" in sections:
+            log("hist leg")
+            line["hist"] = hist_leg(args, stream, runs)
+            log(f"hist: {line['hist']}")
+        del stream, runs, codes
     if "batched" in sections:
         log("batched leg")
         line["batched"] = batched_leg(args)
         log(f"batched: {line['batched']}")
-    if "precision" in sections:
-        log("precision leg")
-        line["precision_form"] = precision_leg(args)
-        log(f"precision: {line['precision_form']}")
     if "serve" in sections:
         log("serve leg")
         line["serve"] = serve_leg(args)
         log(f"serve: {line['serve']}")
+    if "config0" in sections:
+        log("reference CPU config leg")
+        line["config0"] = config0_leg(args)
+        log(f"config0: {line['config0']}")
     if "cli" in sections:
         log("cli leg")
         line["cli"] = cli_leg(args)
         log(f"cli: {line['cli']}")
-    if "hist" in sections:
-        log("hist leg")
-        line["hist"] = hist_leg(args, stream, runs)
-        log(f"hist: {line['hist']}")
     if not args.no_cpu:
         log("cpu baseline leg")
-        line["cpu_baseline"] = cpu_leg(stream, runs, pwms, thr, args.cpu_budget)
+        g = groups[0]
+        line["cpu_baseline"] = cpu_leg(g["codes"], np.zeros(1, np.uint64), g["pwms"], g["thr"], args.cpu_budget,
+                                       f"group {g['name']} of the precision workload")
     print(json.dumps(line), flush=True)
 
 
