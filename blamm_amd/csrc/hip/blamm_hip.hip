@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -119,6 +120,42 @@ struct GroupInfo {
 
 }  // namespace
 
+// Scratch of blamm_hip_scan_device_noalloc: the scan counters (first 256
+// bytes) and BLAMM_NOALLOC_MAX_RECORDS filter records of 16 bytes, 64 MiB in
+// all.  Every GPU the library runs on holds its own copy until the process
+// ends; all contexts on one GPU share it, so no-alloc scans that overlap on
+// one GPU race.  Host code never uses the array's name as an address: the
+// device address of each GPU's copy comes from hipGetSymbolAddress, taken
+// once per device in blamm_hip_create with that device current.
+constexpr uint64_t kNoAllocHeader = 256;
+static_assert(kCntSlots * sizeof(unsigned long long) <= kNoAllocHeader, "counters exceed the scratch header");
+static __device__ __align__(256) uint8_t scratch[kNoAllocHeader + (uint64_t)BLAMM_NOALLOC_MAX_RECORDS * 16];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::string g_scratch_err[kMaxDevices];
+
+// Device address of the current device's scratch (taken once per device).
+void noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) {
+        g_scratch_ptr[device] = p;
+        g_scratch_err[device].clear();
+    } else {
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    }
+}
+
+}  // namespace
+
 struct blamm_hip_ctx {
     int device = 0;
     int num_cu = 256;
@@ -167,6 +204,7 @@ struct blamm_hip_ctx {
 
     // last scan
     bool pending = false;
+    bool pending_noalloc = false;             // records and counters in the no-alloc scratch
     uint64_t pending_hit_cap = 0;
     uint64_t pending_cand_cap = 0;            // record capacity the filter was launched with
     unsigned long long* pending_counters = nullptr;
@@ -558,6 +596,7 @@ int blamm_hip_create(int device, blamm_hip_ctx** out)
         const int v = std::atoi(e);
         if (v >= 1 && v <= 64) c->filter_rounds = v;
     }
+    noalloc_scratch_init(device);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = c->counters.reserve(kCntSlots);
@@ -757,8 +796,32 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     if (c->cand.n == 0) HIP_TRY(c->cand.reserve(1u << 20));
+    c->pending_noalloc = false;
     return enqueue_scan(c, d_packed, d_lcmask, nbases, d_run_starts, nruns, d_hits, cap, d_nhits, c->cand.p,
                         c->cand.n, c->counters.p, s);
+}
+
+int blamm_hip_scan_device_noalloc(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t packed_words,
+                                  const uint64_t* d_lcmask, uint64_t mask_words, uint64_t nbases,
+                                  const uint64_t* d_run_starts, uint64_t nruns, blamm_hit* d_hits,
+                                  uint64_t cap, uint64_t* d_nhits, void* stream)
+{
+    BLAMM_TRY(check_scan_args(c, d_packed, packed_words, d_lcmask, mask_words, nbases, d_run_starts, nruns, d_hits,
+                              cap));
+    if (c->device < 0 || c->device >= kMaxDevices)
+        return fail(BLAMM_ERR_DEVICE, "scan_noalloc: device index beyond the scratch table");
+    uint8_t* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = (uint8_t*)g_scratch_ptr[c->device];
+        if (!base) return fail(BLAMM_ERR_DEVICE, "scan_noalloc: no scratch address (" + g_scratch_err[c->device] + ")");
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    c->pending_noalloc = true;
+    return enqueue_scan(c, d_packed, d_lcmask, nbases, d_run_starts, nruns, d_hits, cap, d_nhits,
+                        (uint4*)(base + kNoAllocHeader), (uint64_t)BLAMM_NOALLOC_MAX_RECORDS,
+                        (unsigned long long*)base, s);
 }
 
 int blamm_hip_scan_finish(blamm_hip_ctx* c, uint64_t* nhits)
@@ -783,6 +846,11 @@ int blamm_hip_scan_finish(blamm_hip_ctx* c, uint64_t* nhits)
         // Filter records were dropped, so the hit count is incomplete: report
         // 0 hits and the overflow.
         if (nhits) *nhits = 0;
+        if (c->pending_noalloc)
+            return fail(BLAMM_ERR_OVERFLOW, "scan_noalloc: " + std::to_string(cnt[kCntCand]) +
+                                                " filter records, more than the no-alloc scratch holds (" +
+                                                std::to_string(c->pending_cand_cap) +
+                                                "); scan a shorter stream or fewer columns per call");
         // Grow the scratch so that a retry holds every candidate.
         HIP_TRY(c->cand.reserve((size_t)(cnt[kCntCand] + cnt[kCntCand] / 4 + 1024)));
         return fail(BLAMM_ERR_OVERFLOW, "scan: candidate scratch overflowed (grown; retry the scan)");

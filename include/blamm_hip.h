@@ -155,14 +155,32 @@ int blamm_hip_scan_device(blamm_hip_ctx *ctx, const uint64_t *d_packed, uint64_t
                           const uint64_t *d_run_starts, uint64_t nruns,
                           blamm_hit *d_hits, uint64_t cap, uint64_t *d_nhits, void *stream);
 
-/* Synchronise the last scan_device.  On success *nhits = hits produced.
- * BLAMM_ERR_OVERFLOW has two causes:
+/* Synchronise the last scan_device (or scan_device_noalloc).  On success
+ * *nhits = hits produced.  BLAMM_ERR_OVERFLOW has two causes:
  *   - the hit buffer was too small: *nhits = the required capacity;
  *   - the filter records exceeded the candidate scratch: *nhits = 0 (the hit
  *     count is incomplete); after scan_device the scratch has been grown, so
  *     a retry succeeds (blamm_hip_last_stats().candidates holds the count).
  * Allocates only to grow the candidate scratch of scan_device. */
 int blamm_hip_scan_finish(blamm_hip_ctx *ctx, uint64_t *nhits);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): blamm_hip_scan_device with its scratch -- the scan
+ * counters and the filter records -- in a static device array of the library
+ * (64 MiB per GPU, held until the process ends) instead of context buffers.
+ * It calls no hipMalloc/hipFree, nor does blamm_hip_scan_finish after it.
+ * Arguments, result and completion (blamm_hip_scan_finish) are those of
+ * blamm_hip_scan_device.  A scan whose filter records exceed
+ * BLAMM_NOALLOC_MAX_RECORDS is rejected by scan_finish with
+ * BLAMM_ERR_OVERFLOW and *nhits = 0 (nothing is grown; split the stream).
+ * Every context on one GPU shares that GPU's array, which keeps the last
+ * call's contents: no-alloc scans that overlap on one GPU race. */
+#define BLAMM_NOALLOC_MAX_RECORDS 4194304u
+int blamm_hip_scan_device_noalloc(blamm_hip_ctx *ctx, const uint64_t *d_packed, uint64_t packed_words,
+                                  const uint64_t *d_lcmask, uint64_t mask_words, uint64_t nbases,
+                                  const uint64_t *d_run_starts, uint64_t nruns,
+                                  blamm_hit *d_hits, uint64_t cap, uint64_t *d_nhits, void *stream);
 
 /* Counters of the last finished scan. */
 int blamm_hip_last_stats(blamm_hip_ctx *ctx, blamm_scan_stats *stats);
