@@ -215,14 +215,14 @@ __device__ __forceinline__ uint32_t hist_range(const HistArgs& a, Stage& st, uin
         }
         // every window of the tile fits its run for every length <= 4 NCH
         const bool fast = q0 + (kBf16Tile - 1) + 4 * NCH <= rnext && q0 + kBf16Tile <= a.qend;
-        mfma_result_guard();
+        const f32x16 (&av)[NB] = acc_read<AccRead::kGuard>(acc);
         const uint32_t qlane = q0 + 4 * (lane >> 5);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const uint32_t defer =
-                fast ? hist_rows<true, LDS>(a, hl, acc[b], iw[b], cc[b], hm[b], len[b], slot[b], hoff[b], qlane, jr,
+                fast ? hist_rows<true, LDS>(a, hl, av[b], iw[b], cc[b], hm[b], len[b], slot[b], hoff[b], qlane, jr,
                                             rnext)
-                     : hist_rows<false, LDS>(a, hl, acc[b], iw[b], cc[b], hm[b], len[b], slot[b], hoff[b], qlane, jr,
+                     : hist_rows<false, LDS>(a, hl, av[b], iw[b], cc[b], hm[b], len[b], slot[b], hoff[b], qlane, jr,
                                              rnext);
             const uint64_t m = __ballot(defer != 0u);
             if (m) {

@@ -179,6 +179,20 @@ __device__ __forceinline__ void mfma_result_guard()
     __builtin_amdgcn_sched_barrier(0);
 }
 
+// Every VALU read of an MFMA accumulator goes through acc_read: kGuard places
+// the guard's wait states first; kCovered states, at the call, that the
+// MFMAs of a whole further tile were issued after the last MFMA that wrote
+// the accumulator (the software-pipelined loops), which covers the result
+// latency many times over.
+enum class AccRead { kGuard, kCovered };
+
+template <AccRead R, typename V>
+__device__ __forceinline__ const V& acc_read(const V& acc)
+{
+    if constexpr (R == AccRead::kGuard) mfma_result_guard();
+    return acc;
+}
+
 // 64 bits of the 128-bit value (hi:lo) starting at bit s (0 <= s < 64),
 // without a branch for s == 0.
 __device__ __forceinline__ uint64_t funnel64(uint64_t lo, uint64_t hi, uint32_t s, uint32_t s_inv)
@@ -341,18 +355,19 @@ __device__ __forceinline__ float tile_max(const f32x16& v)
     return mx;
 }
 
-// Finish a tile whose hi products are in acc (read no sooner than the MFMAs
-// allow: callers issue other MFMAs or a guard in between).  Lazy-lo form: the
-// lo fragments come from the wave's LDS slice.
-template <int NCH, int NB, bool MASK, bool LO>
+// Finish a tile whose hi products are in acc (R: how the first read is kept
+// clear of the matrix cores).  Lazy-lo form: the lo fragments come from the
+// wave's LDS slice.
+template <int NCH, int NB, bool MASK, bool LO, AccRead R>
 __device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st, f32x16 (&acc)[NB],
                                                  const float (&th)[NB], const float (&thh)[NB], const OneHotSrc& bits,
                                                  uint64_t mb, uint32_t q0, uint32_t blk0, uint32_t lane)
 {
     uint32_t need = 0;
+    const f32x16 (&hv)[NB] = acc_read<R>(acc);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-        if (__ballot(tile_max<NB>(acc[b]) >= thh[b])) need |= 1u << b;
+        if (__ballot(tile_max<NB>(hv[b]) >= thh[b])) need |= 1u << b;
     if (need == 0) return;
     const uint32_t qlane = q0 + 4 * (lane >> 5);
     if constexpr (!LO) {
@@ -362,7 +377,7 @@ __device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st,
             if (need & (1u << b)) {
                 float v[16];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) v[r] = acc[b][r];
+                for (int r = 0; r < 16; ++r) v[r] = hv[b][r];
                 emit_rows<16>(a, st, v, thh[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
             }
     } else {
@@ -378,13 +393,13 @@ __device__ __forceinline__ void bf16_finish_tile(const FilterArgs& a, Stage& st,
             for (int j = 0; j < NCH; ++j)
                 acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(onehot_chunk<MASK>(bits, mb, j), B[j], acc[b], 0, 0, 0);
         }
-        mfma_result_guard();
+        const f32x16 (&fv)[NB] = acc_read<AccRead::kGuard>(acc);
 #pragma unroll
         for (int b = 0; b < NB; ++b)
             if (need & (1u << b)) {
                 float v[16];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) v[r] = acc[b][r];
+                for (int r = 0; r < 16; ++r) v[r] = fv[b][r];
                 emit_rows<16>(a, st, v, th[b], qlane, (blk0 + b) * kBf16Cols + (lane & 31));
             }
     }
@@ -442,11 +457,12 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         bB = onehot_src<NCH>(w0, w1, w2, sh);
         if (MASK) mB = window_mask<NCH>(m0, m1, m2, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
-        bf16_finish_tile<NCH, NB, MASK, LO>(a, st, accA, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
+        bf16_finish_tile<NCH, NB, MASK, LO, AccRead::kCovered>(a, st, accA, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0,
+                                                              lane);
         bA = onehot_src<NCH>(w1, w2, w3, sh);
         if (MASK) mA = window_mask<NCH>(m1, m2, m3, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accA, bh, bA, mA);
-        bf16_finish_tile<NCH, NB, MASK, LO>(a, st, accB, th, thh, bB, mB, t * kBf16Tile, blk0, lane);
+        bf16_finish_tile<NCH, NB, MASK, LO, AccRead::kCovered>(a, st, accB, th, thh, bB, mB, t * kBf16Tile, blk0, lane);
         w0 = w2; w1 = w3; w2 = w4;
         if (MASK) { m0 = m2; m1 = m3; m2 = m4; }
     }
@@ -454,15 +470,14 @@ __device__ __forceinline__ void bf16_range(const FilterArgs& a, Stage& st, uint3
         bB = onehot_src<NCH>(w0, w1, w2, sh);
         if (MASK) mB = window_mask<NCH>(m0, m1, m2, off & 31);
         bf16_hi_tile<NCH, NB, MASK>(accB, bh, bB, mB);
-        bf16_finish_tile<NCH, NB, MASK, LO>(a, st, accA, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
+        bf16_finish_tile<NCH, NB, MASK, LO, AccRead::kCovered>(a, st, accA, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0,
+                                                              lane);
         pendingA = false;
-        mfma_result_guard();
-        bf16_finish_tile<NCH, NB, MASK, LO>(a, st, accB, th, thh, bB, mB, t * kBf16Tile, blk0, lane);
+        bf16_finish_tile<NCH, NB, MASK, LO, AccRead::kGuard>(a, st, accB, th, thh, bB, mB, t * kBf16Tile, blk0, lane);
     }
-    if (pendingA) {
-        mfma_result_guard();
-        bf16_finish_tile<NCH, NB, MASK, LO>(a, st, accA, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0, lane);
-    }
+    if (pendingA)
+        bf16_finish_tile<NCH, NB, MASK, LO, AccRead::kGuard>(a, st, accA, th, thh, bA, mA, (t - 1) * kBf16Tile, blk0,
+                                                            lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -492,10 +507,10 @@ __device__ __forceinline__ void fp32_subtile(const FilterArgs& a, Stage& st, con
             acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bw[b][p], acc[b], 0, 0, 0);
     }
     const uint32_t qlane = q0 + 4 * k;     // windows qlane + 0..3
-    mfma_result_guard();
+    const f32x4 (&av)[NB] = acc_read<AccRead::kGuard>(acc);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const float v[4] = {acc[b][0], acc[b][1], acc[b][2], acc[b][3]};
+        const float v[4] = {av[b][0], av[b][1], av[b][2], av[b][3]};
         emit_rows<4>(a, st, v, th[b], qlane, (blk0 + b) * kFp32Cols + (lane & 15));
     }
 }
@@ -909,10 +924,12 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
     flush();
 }
 
+#ifndef BLAMM_INSTANTIATE   // small kernels: launched from the main unit only
 static __global__ void zero_counters_kernel(unsigned long long* c)
 {
     if (threadIdx.x < kCntSlots) c[threadIdx.x] = 0ull;
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Hits of a batch grouped by request on the device (count, scan, scatter), so
@@ -936,6 +953,7 @@ struct GroupArgs {
     const uint32_t* req_start;            // [nreq] first batch position of each request
 };
 
+#ifndef BLAMM_INSTANTIATE
 static __global__ __launch_bounds__(kGroupThreads) void group_clear_kernel(GroupArgs g)
 {
     const uint32_t nreq = g.desc->nreq;
@@ -1034,6 +1052,7 @@ static __global__ __launch_bounds__(kGroupThreads) void group_scatter_kernel(Gro
         g.out[at] = h;
     }
 }
+#endif  // BLAMM_INSTANTIATE
 static_assert(kCntQueue + kXcds <= kCntSlots, "queue counters");
 
 // Column blocks of a batch: batch slot s takes the fragments of library slot
