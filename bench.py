@@ -472,7 +472,7 @@ def serve_leg(args):
         f"{sum(r.nbases for r, _ in pool) / len(pool) / 1e6:.2f} Mbp per request ({time.time() - t0:.1f}s)")
     sc = ba.HipScanner(0)
     sc.set_library(P, L, thr[1e-4], ba.PREC_BF16X2)
-    srv = ba.Server(sc, hbm_budget_bytes=args.serve_budget_gib << 30, slots=2, max_batch_requests=64,
+    srv = ba.Server(sc, hbm_budget_bytes=args.serve_budget_gib << 30, slots=args.serve_slots, max_batch_requests=64,
                     max_batch_bases=args.serve_batch_mbp * 1_000_000)
 
     def run(arrivals):
@@ -538,7 +538,10 @@ def serve_leg(args):
         log(f"serve load {f:.2f}: {out['poisson'][-1]}")
     st = srv.stats()
     out.update({"batches": int(st.batches), "replays": int(st.replays), "device_ms": st.device_ms,
-                "host_submit_ms": st.host_submit_ms, "host_wait_ms": st.host_wait_ms})
+                "host_submit_ms": st.host_submit_ms, "host_wait_ms": st.host_wait_ms, "hits": int(st.hits),
+                "slots": args.serve_slots, "batch_mbp": args.serve_batch_mbp,
+                "pool_mbp_per_request": sum(r.nbases for r, _ in pool) / len(pool) / 1e6,
+                "pool_gbp_motifs_per_request": sum(w for _, w in pool) / len(pool) / 1e9})
     srv.close()
     sc.close()
     return out
@@ -792,6 +795,7 @@ def main():
     ap.add_argument("--serve-budget-gib", type=int, default=32)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-batch-mbp", type=int, default=16, help="bases per serving batch (Mbp)")
+    ap.add_argument("--serve-slots", type=int, default=3, help="serving batches in flight")
     ap.add_argument("--serve-loads", type=lambda v: [float(x) for x in v.split(",")], default=[0.3, 0.6, 0.85])
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for scaling FETCH_SIZE/WRITE_SIZE)")

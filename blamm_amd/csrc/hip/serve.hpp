@@ -1,12 +1,16 @@
 // Serving loop: a resident library, requests admitted by an HBM budget and
-// batched as they arrive (continuous batching).  A worker thread owns
-// `slots` batch slots; whenever a slot is free it takes the queued requests
-// (first come, first served) that fit the slot's share of the budget and
-// launches them as one batch, while the other slots' batches are still on
-// the device.  Finished batches are harvested in completion order and their
-// hits handed to the requests.  Every request's result is the same as if it
-// had been scanned alone: thresholds, run rule and scores are per request
-// and per window.
+// batched as they arrive (continuous batching).  The server owns `slots`
+// batch slots and two threads:
+//  - the launcher takes a free slot whenever one exists, packs the queued
+//    requests (first come, first served) that fit the slot's share of the
+//    budget into one batch and launches it while other slots' batches are
+//    still on the device;
+//  - the harvester waits for the oldest batch in flight, copies its hits
+//    (grouped by request) into pinned memory and hands them to the requests,
+//    then frees the slot.
+// So a batch's hit read-back never holds up the launch of the next one.
+// Every request's result is the same as if it had been scanned alone:
+// thresholds, run rule and scores are per request and per window.
 //
 // Included once, at the end of blamm_hip.hip.
 
@@ -40,7 +44,8 @@ struct ServeRequest {
 struct ServeSlot {
     blamm_hip_batch* batch = nullptr;
     std::vector<uint64_t> ids;       // requests of the batch in flight
-    bool busy = false;
+    // free -> launching (launcher) -> in flight -> harvesting (harvester) -> free
+    enum State { kFree, kLaunching, kInFlight, kHarvesting } state = kFree;
     uint64_t launched = 0;           // launch order, for harvesting oldest first
 };
 
@@ -52,12 +57,14 @@ struct blamm_hip_server {
     uint64_t slot_budget = 0;
     std::vector<ServeSlot> slots;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    // cv_work: a request was queued or a slot was freed (launcher);
+    // cv_flight: a batch was launched (harvester); cv_done: requests finished
+    std::condition_variable cv_work, cv_flight, cv_done;
     std::deque<uint64_t> queue;
     std::unordered_map<uint64_t, ServeRequest> reqs;
     uint64_t next_id = 1, launches = 0;
     bool stop = false;
-    std::thread worker;
+    std::thread launcher, harvester;
     blamm_server_stats stats{};
 };
 
@@ -102,92 +109,101 @@ void serve_finish(blamm_hip_server* S, ServeSlot& sl, int rc, std::unique_lock<s
         S->stats.hits += q.nhits;
     }
     sl.ids.clear();
-    sl.busy = false;
+    sl.state = ServeSlot::kFree;
     (void)lk;
     S->cv_done.notify_all();
+    S->cv_work.notify_all();
 }
 
-void serve_loop(blamm_hip_server* S)
+// Launcher thread: fill free slots with batches of queued requests.
+void serve_launch_loop(blamm_hip_server* S)
 {
     (void)hipSetDevice(S->ctx->device);
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
-        bool progressed = false;
-        // 1. harvest finished batches, oldest first
-        for (;;) {
-            ServeSlot* oldest = nullptr;
-            for (ServeSlot& sl : S->slots)
-                if (sl.busy && (!oldest || sl.launched < oldest->launched)) oldest = &sl;
-            if (!oldest || blamm_hip_batch_query(oldest->batch) == 1) break;
-            lk.unlock();
-            uint64_t nh = 0;
-            const uint64_t t0 = now_ns();
-            const uint64_t rp0 = oldest->batch->replays;
-            const int rc = blamm_hip_batch_wait(oldest->batch, &nh);
-            blamm_scan_stats bs{};
-            blamm_hip_batch_stats(oldest->batch, &bs);
-            const uint64_t t1 = now_ns();
-            lk.lock();
-            S->stats.device_ms += bs.kernel_ms;
-            S->stats.host_wait_ms += (t1 - t0) / 1e6;
-            S->stats.replays += oldest->batch->replays - rp0;
-            serve_finish(S, *oldest, rc, lk);
-            progressed = true;
+        ServeSlot* sl = nullptr;
+        S->cv_work.wait(lk, [&] {
+            if (S->stop && S->queue.empty()) return true;
+            if (S->queue.empty()) return false;
+            for (ServeSlot& x : S->slots)
+                if (x.state == ServeSlot::kFree) {
+                    sl = &x;
+                    return true;
+                }
+            return false;
+        });
+        if (!sl) break;   // stopping with an empty queue
+        std::vector<uint64_t> ids;
+        std::vector<blamm_request> batch;
+        uint64_t bytes = 0, bases = 0;
+        while (!S->queue.empty() && ids.size() < S->cfg.max_batch_requests) {
+            const ServeRequest& q = S->reqs[S->queue.front()];
+            if (!ids.empty() && (bytes + q.bytes > S->slot_budget ||
+                                 bases + q.req.nbases + kReqAlign > S->cfg.max_batch_bases))
+                break;
+            bytes += q.bytes;
+            bases += q.req.nbases + kReqAlign;
+            ids.push_back(S->queue.front());
+            batch.push_back(q.req);
+            S->queue.pop_front();
         }
-        // 2. launch queued requests on free slots
-        for (ServeSlot& sl : S->slots) {
-            if (sl.busy || S->queue.empty()) continue;
-            std::vector<uint64_t> ids;
-            std::vector<blamm_request> batch;
-            uint64_t bytes = 0, bases = 0;
-            while (!S->queue.empty() && ids.size() < S->cfg.max_batch_requests) {
-                const ServeRequest& q = S->reqs[S->queue.front()];
-                if (!ids.empty() && (bytes + q.bytes > S->slot_budget ||
-                                     bases + q.req.nbases + kReqAlign > S->cfg.max_batch_bases))
-                    break;
-                bytes += q.bytes;
-                bases += q.req.nbases + kReqAlign;
-                ids.push_back(S->queue.front());
-                batch.push_back(q.req);
-                S->queue.pop_front();
-            }
-            const uint64_t t = now_ns();
-            for (uint64_t id : ids) {
-                S->reqs[id].t_start = t;
-                S->reqs[id].batch_requests = (uint32_t)ids.size();
-            }
-            sl.ids = ids;
-            sl.busy = true;
-            sl.launched = ++S->launches;
-            S->stats.batches++;
-            lk.unlock();
-            const int rc = blamm_hip_batch_submit(sl.batch, batch.data(), (uint32_t)batch.size());
-            const uint64_t t1 = now_ns();
-            lk.lock();
-            S->stats.host_submit_ms += (t1 - t) / 1e6;
-            if (rc != BLAMM_OK) serve_finish(S, sl, rc, lk);
-            progressed = true;
+        const uint64_t t = now_ns();
+        for (uint64_t id : ids) {
+            S->reqs[id].t_start = t;
+            S->reqs[id].batch_requests = (uint32_t)ids.size();
         }
-        bool busy = false, idle = false;
-        for (const ServeSlot& sl : S->slots) (sl.busy ? busy : idle) = true;
-        if (S->stop && S->queue.empty() && !busy) break;
-        if (progressed) continue;
-        if (busy && !idle) {
-            // every slot is on the device: block until the oldest batch ends
-            ServeSlot* oldest = nullptr;
-            for (ServeSlot& sl : S->slots)
-                if (sl.busy && (!oldest || sl.launched < oldest->launched)) oldest = &sl;
-            hipEvent_t ev = oldest->batch->ev_done;
-            lk.unlock();
-            (void)hipEventSynchronize(ev);
-            lk.lock();
-        } else if (busy) {
-            // a batch is running and a slot is free: wake on a new request or
-            // poll the running batch
-            S->cv_work.wait_for(lk, std::chrono::microseconds(100));
+        sl->ids = ids;
+        sl->state = ServeSlot::kLaunching;
+        S->stats.batches++;
+        lk.unlock();
+        const int rc = blamm_hip_batch_submit(sl->batch, batch.data(), (uint32_t)batch.size());
+        const uint64_t t1 = now_ns();
+        lk.lock();
+        S->stats.host_submit_ms += (t1 - t) / 1e6;
+        if (rc != BLAMM_OK) {
+            serve_finish(S, *sl, rc, lk);
+            S->cv_flight.notify_all();
         } else {
-            S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+            sl->launched = ++S->launches;
+            sl->state = ServeSlot::kInFlight;
+            S->cv_flight.notify_all();
         }
+    }
+    S->cv_flight.notify_all();
+}
+
+// Harvester thread: finish batches oldest first (wait, hit read-back,
+// hand-out), freeing their slots.
+void serve_harvest_loop(blamm_hip_server* S)
+{
+    (void)hipSetDevice(S->ctx->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        ServeSlot* oldest = nullptr;
+        S->cv_flight.wait(lk, [&] {
+            oldest = nullptr;
+            bool launching = false;
+            for (ServeSlot& x : S->slots) {
+                if (x.state == ServeSlot::kInFlight && (!oldest || x.launched < oldest->launched)) oldest = &x;
+                launching = launching || x.state == ServeSlot::kLaunching;
+            }
+            return oldest != nullptr || (S->stop && S->queue.empty() && !launching);
+        });
+        if (!oldest) break;
+        oldest->state = ServeSlot::kHarvesting;
+        lk.unlock();
+        uint64_t nh = 0;
+        const uint64_t t0 = now_ns();
+        const uint64_t rp0 = oldest->batch->replays;
+        const int rc = blamm_hip_batch_wait(oldest->batch, &nh);
+        blamm_scan_stats bs{};
+        blamm_hip_batch_stats(oldest->batch, &bs);
+        const uint64_t t1 = now_ns();
+        lk.lock();
+        S->stats.device_ms += bs.kernel_ms;
+        S->stats.host_wait_ms += (t1 - t0) / 1e6;
+        S->stats.replays += oldest->batch->replays - rp0;
+        serve_finish(S, *oldest, rc, lk);
     }
 }
 
@@ -223,7 +239,8 @@ int blamm_hip_server_create(blamm_hip_ctx* c, const blamm_server_config* cfg, bl
             return rc;
         }
     }
-    S->worker = std::thread(serve_loop, S);
+    S->launcher = std::thread(serve_launch_loop, S);
+    S->harvester = std::thread(serve_harvest_loop, S);
     *out = S;
     return BLAMM_OK;
 }
@@ -344,7 +361,9 @@ int blamm_hip_server_destroy(blamm_hip_server* S)
         S->stop = true;
     }
     S->cv_work.notify_all();
-    if (S->worker.joinable()) S->worker.join();
+    S->cv_flight.notify_all();
+    if (S->launcher.joinable()) S->launcher.join();
+    if (S->harvester.joinable()) S->harvester.join();
     for (ServeSlot& sl : S->slots) blamm_hip_batch_destroy(sl.batch);
     delete S;
     return BLAMM_OK;

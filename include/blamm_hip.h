@@ -296,13 +296,15 @@ typedef struct blamm_server_stats {
     uint64_t submitted, completed, batches, hits;
     uint64_t replays;             /* batches run again after growing scratch */
     double device_ms;             /* summed batch launch times */
-    double host_submit_ms;        /* worker time staging and launching batches */
-    double host_wait_ms;          /* worker time in waits and hit read-back */
+    double host_submit_ms;        /* launcher thread: staging and launching batches */
+    double host_wait_ms;          /* harvester thread: waits and hit read-back */
 } blamm_server_stats;
 
 typedef struct blamm_hip_server blamm_hip_server;
 
-/* Starts a worker thread that owns the batch slots.  While the server runs,
+/* Starts the server's two threads (a launcher that packs queued requests into
+ * free batch slots, a harvester that finishes batches oldest first and reads
+ * their hits back), which own the batch slots.  While the server runs,
  * the context's library and tuning must not change. */
 int blamm_hip_server_create(blamm_hip_ctx *ctx, const blamm_server_config *cfg,
                             blamm_hip_server **server);
@@ -322,7 +324,7 @@ int blamm_hip_server_result(blamm_hip_server *server, uint64_t id, blamm_hit *hi
  * it (for callers that only need the count). */
 int blamm_hip_server_release(blamm_hip_server *server, uint64_t id, blamm_request_info *info);
 int blamm_hip_server_stats(blamm_hip_server *server, blamm_server_stats *stats);
-/* Finishes queued and running requests, then stops the worker. */
+/* Finishes queued and running requests, then stops the server's threads. */
 int blamm_hip_server_destroy(blamm_hip_server *server);
 
 /* ------------------------------------------------------------------------
