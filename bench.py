@@ -461,6 +461,8 @@ def serve_leg(args):
     pool = []
     for _ in range(args.serve_pool):
         codes, starts, subset, p = synth.config4_request(rng, len(ms.pfms), max_seqs=args.serve_max_seqs)
+        if args.serve_pvalue:   # one p-value for every request (same pool otherwise)
+            p = args.serve_pvalue
         t = np.full(ncols, np.inf, np.float32)
         cols = np.concatenate([2 * subset, 2 * subset + 1])
         t[cols] = thr[p][cols]
@@ -515,7 +517,9 @@ def serve_leg(args):
     out = {
         "workload": f"serving config: library {ncols // 2} motifs (L 5-30, Dirichlet 0.3) fwd+rc = {ncols} columns; "
                     f"requests of 1-{args.serve_max_seqs} sequences (lognormal median 1 kbp, sigma 1, max 1 Mbp), "
-                    f"motif subsets U{{10..2000}}, p in {{1e-3,1e-4,1e-5,1e-6}}",
+                    f"motif subsets U{{10..2000}}, " +
+                    (f"p = {args.serve_pvalue:g} for every request" if args.serve_pvalue else
+                     "p in {1e-3,1e-4,1e-5,1e-6}"),
         "unit": "Gbp*motifs/s",
         "closed_loop": {"requests": n_closed, "rps": cap_rps, "value": work / span / 1e9,
                         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
@@ -796,6 +800,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-batch-mbp", type=int, default=16, help="bases per serving batch (Mbp)")
     ap.add_argument("--serve-slots", type=int, default=3, help="serving batches in flight")
+    ap.add_argument("--serve-pvalue", type=float, default=None,
+                    help="give every serving request this p-value (1e-3..1e-6) instead of drawing it")
     ap.add_argument("--serve-loads", type=lambda v: [float(x) for x in v.split(",")], default=[0.3, 0.6, 0.85])
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for scaling FETCH_SIZE/WRITE_SIZE)")

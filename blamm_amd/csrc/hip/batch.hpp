@@ -23,7 +23,7 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // and returns to the pool when the last of them lets go, so a busy server
 // recycles pinned memory instead of allocating and freeing it.
 struct HitBuf {
-    blamm_hit* p = nullptr;
+    GroupedHit* p = nullptr;
     size_t n = 0;
 };
 
@@ -35,6 +35,16 @@ struct HitPool {
         for (HitBuf& h : free) (void)hipHostFree(h.p);
     }
 };
+
+// GroupedHit (request-local 32-bit position) -> blamm_hit.
+void expand_hits(const GroupedHit* in, uint64_t n, blamm_hit* out)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        out[i].pos = in[i].pos;
+        out[i].col = in[i].col;
+        out[i].score = in[i].score;
+    }
+}
 
 int acquire_hits(const std::shared_ptr<HitPool>& pool, size_t want, std::shared_ptr<HitBuf>* out)
 {
@@ -52,7 +62,7 @@ int acquire_hits(const std::shared_ptr<HitPool>& pool, size_t want, std::shared_
     }
     if (!got.p) {
         got.n = std::max<size_t>(want + want / 4, 1u << 16);
-        hipError_t e = hipHostMalloc((void**)&got.p, got.n * sizeof(blamm_hit), hipHostMallocDefault);
+        hipError_t e = hipHostMalloc((void**)&got.p, got.n * sizeof(GroupedHit), hipHostMallocDefault);
         if (e != hipSuccess)
             return fail(e == hipErrorOutOfMemory ? BLAMM_ERR_NOMEM : BLAMM_ERR_DEVICE,
                         std::string("pinned hit buffer: ") + hipGetErrorString(e));
@@ -68,37 +78,65 @@ int acquire_hits(const std::shared_ptr<HitPool>& pool, size_t want, std::shared_
     return BLAMM_OK;
 }
 
+// Host-written tables of a batch live in one device arena laid out like their
+// pinned staging image, so that one copy uploads them all.  Each table has a
+// capacity; a batch that needs more grows the table geometrically (the arena
+// moves and the captured graphs are dropped).  The work items come last, so
+// the copy stops at the end of the batch's items.
+enum MetaTable {
+    kMetaDesc, kMetaReqStart, kMetaReqEnd, kMetaThr, kMetaSlotMap, kMetaBlkReq, kMetaGroups, kMetaChunks,
+    kMetaRuns, kMetaItems, kMetaTables
+};
+constexpr size_t kMetaElem[kMetaTables] = {sizeof(BatchDesc), 4, 4, 4, 4, 4, 4, 16, 8, 8};
+
+template <typename T>
+struct MetaView {
+    T* p = nullptr;
+};
+
 }  // namespace
 
 struct blamm_hip_batch {
     blamm_hip_ctx* ctx = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_done = nullptr;
+    hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_done = nullptr, ev_copy = nullptr;
 
     // device buffers
-    DevBuf<uint64_t> packed, lcmask, runs;
+    DevBuf<uint64_t> packed, lcmask, run_bits;
+    // host-written tables (views into `meta`)
+    DevBuf<uint8_t> meta;
+    size_t meta_cap[kMetaTables] = {};
+    size_t meta_off[kMetaTables + 1] = {};
+    MetaView<BatchDesc> desc;
+    MetaView<uint32_t> req_start, req_end;
     // the batch's own column blocks: each request's selected columns
-    DevBuf<float> thr;              // per batch slot: the request's threshold
-    DevBuf<uint32_t> slot_map;      // per batch slot: library slot (kNoSlot = empty)
-    DevBuf<uint32_t> blk_req;       // per batch block: request
-    DevBuf<uint32_t> groups;        // column groups (group_pack)
+    MetaView<float> thr;            // per batch slot: the request's threshold
+    MetaView<uint32_t> slot_map;    // per batch slot: library slot (kNoSlot = empty)
+    MetaView<uint32_t> blk_req;     // per batch block: request
+    MetaView<uint32_t> groups;      // column groups (group_pack)
+    MetaView<uint4> chunks;         // (t0, t1, request, 0)
+    MetaView<uint64_t> runs;        // run starts of the batch stream
+    MetaView<uint2> items;          // (chunk, group), per XCD sub-queue
+    // device-written
     DevBuf<uint4> frag_a, frag_b;   // gathered fragments (fp32 weights in frag_a)
     DevBuf<double> band, band_hi;   // per batch slot
-    DevBuf<uint32_t> req_start, req_end, req_count;
+    DevBuf<uint32_t> req_count;
     DevBuf<unsigned long long> req_off, cursor;
-    DevBuf<uint2> items;
     DevBuf<uint4> cand;
-    DevBuf<uint4> chunks;
-    DevBuf<BatchDesc> desc;
     DevBuf<unsigned long long> counters;
-    DevBuf<blamm_hit> hits, grouped;
+    DevBuf<DevHit> hits;            // finalize output
+    DevBuf<GroupedHit> grouped;     // grouped by request, read back
     uint64_t buf_gen = 1;           // bumped whenever a device buffer moves
 
     // pinned staging
     HostBuf<uint8_t> stage;
     HostBuf<unsigned long long> cnt_h, off_h;
     std::shared_ptr<HitPool> pool = std::make_shared<HitPool>();
+    std::shared_ptr<HitBuf> copying;    // the batch's hits while they are copied back
+    uint64_t copy_nh = 0;
     std::shared_ptr<HitBuf> out;    // hits of the last finished batch, grouped by request
+    std::vector<blamm_hit> expanded;    // out as blamm_hit, built by the first batch_hits call
+    bool have_expanded = false;
 
     // captured launch sequence per lowercase variant
     hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -125,13 +163,47 @@ void drop_graphs(blamm_hip_batch* b)
     }
 }
 
-// Grow a device buffer; a move invalidates the captured graphs.
+// Grow a device buffer (by at least half, so a slot serving batches of
+// varying size settles after a few growths); a move invalidates the captured
+// graphs.
 template <typename T>
 int batch_grow(blamm_hip_batch* b, DevBuf<T>& buf, size_t want)
 {
     if (want <= buf.n && buf.p) return BLAMM_OK;
     HIP_TRY(hipStreamSynchronize(b->stream));
-    HIP_TRY(buf.reserve(want + want / 8));
+    HIP_TRY(buf.reserve(std::max(want + want / 4, buf.n + buf.n / 2)));
+    b->buf_gen++;
+    return BLAMM_OK;
+}
+
+// Make room for `need` elements of every host-written table.
+int meta_fit(blamm_hip_batch* b, const size_t (&need)[kMetaTables])
+{
+    bool grow = !b->meta.p;
+    for (int x = 0; x < kMetaTables; ++x) grow = grow || need[x] > b->meta_cap[x];
+    if (!grow) return BLAMM_OK;
+    for (int x = 0; x < kMetaTables; ++x)
+        if (need[x] > b->meta_cap[x] || b->meta_cap[x] == 0)
+            b->meta_cap[x] = std::max<size_t>({need[x] + need[x] / 4, b->meta_cap[x] + b->meta_cap[x] / 2, 1});
+    size_t off = 0;
+    for (int x = 0; x < kMetaTables; ++x) {
+        b->meta_off[x] = off;
+        off = align_up(off + b->meta_cap[x] * kMetaElem[x], 256);
+    }
+    b->meta_off[kMetaTables] = off;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(b->meta.reserve(off));
+    uint8_t* m = b->meta.p;
+    b->desc.p = (BatchDesc*)(m + b->meta_off[kMetaDesc]);
+    b->req_start.p = (uint32_t*)(m + b->meta_off[kMetaReqStart]);
+    b->req_end.p = (uint32_t*)(m + b->meta_off[kMetaReqEnd]);
+    b->thr.p = (float*)(m + b->meta_off[kMetaThr]);
+    b->slot_map.p = (uint32_t*)(m + b->meta_off[kMetaSlotMap]);
+    b->blk_req.p = (uint32_t*)(m + b->meta_off[kMetaBlkReq]);
+    b->groups.p = (uint32_t*)(m + b->meta_off[kMetaGroups]);
+    b->chunks.p = (uint4*)(m + b->meta_off[kMetaChunks]);
+    b->runs.p = (uint64_t*)(m + b->meta_off[kMetaRuns]);
+    b->items.p = (uint2*)(m + b->meta_off[kMetaItems]);
     b->buf_gen++;
     return BLAMM_OK;
 }
@@ -141,6 +213,24 @@ int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
     blamm_hip_ctx* c = b->ctx;
     const bool bf = prec_bf(c->precision);
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, b->stream, b->counters.p);
+    GroupArgs g;
+    g.desc = b->desc.p;
+    g.counters = b->counters.p;
+    g.cap = b->hits.n;
+    g.out_cap = b->grouped.n;
+    g.in = b->hits.p;
+    g.out = b->grouped.p;
+    g.req_count = b->req_count.p;
+    g.req_off = b->req_off.p;
+    g.cursor = b->cursor.p;
+    g.req_start = b->req_start.p;
+    // finalize counts hits per request into req_count
+    hipLaunchKernelGGL(group_clear_kernel, dim3(64), dim3(kGroupThreads), 0, b->stream, g);
+    // run-start bits of the batch stream (request starts included)
+    hipLaunchKernelGGL(run_bits_clear_kernel, dim3((uint32_t)c->num_cu * 4), dim3(256), 0, b->stream, b->run_bits.p,
+                       0u, (const BatchDesc*)b->desc.p);
+    hipLaunchKernelGGL(run_bits_set_kernel, dim3((uint32_t)c->num_cu), dim3(256), 0, b->stream, b->run_bits.p,
+                       (const uint64_t*)b->runs.p, 0u, 0u, (const BatchDesc*)b->desc.p);
     HIP_TRY(hipGetLastError());
     // the batch's column blocks from the library's fragments
     GatherArgs ga;
@@ -184,22 +274,12 @@ int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
     f.req_end = b->req_end.p;
     f.slot_map = b->slot_map.p;
     f.blk_req = b->blk_req.p;
+    f.req_count = b->req_count.p;
+    f.run_bits = b->run_bits.p;
     e = launch_finalize(bf, true, f, (uint32_t)c->num_cu * 8, b->stream);
     if (e != hipSuccess) return fail(BLAMM_ERR_DEVICE, std::string("batch finalize launch: ") + hipGetErrorString(e));
     // group the hits by request on the device
-    GroupArgs g;
-    g.desc = b->desc.p;
-    g.counters = b->counters.p;
-    g.cap = std::min(b->hits.n, b->grouped.n);
-    g.in = (const DevHit*)b->hits.p;
-    g.out = (DevHit*)b->grouped.p;
-    g.req_count = b->req_count.p;
-    g.req_off = b->req_off.p;
-    g.cursor = b->cursor.p;
-    g.req_start = b->req_start.p;
     const uint32_t gg = (uint32_t)c->num_cu * 4;
-    hipLaunchKernelGGL(group_clear_kernel, dim3(64), dim3(kGroupThreads), 0, b->stream, g);
-    hipLaunchKernelGGL(group_count_kernel, dim3(gg), dim3(kGroupThreads), 0, b->stream, g);
     hipLaunchKernelGGL(group_scan_kernel, dim3(1), dim3(1024), 0, b->stream, g);
     hipLaunchKernelGGL(group_scatter_kernel, dim3(gg), dim3(kGroupThreads), 0, b->stream, g);
     e = hipGetLastError();
@@ -248,6 +328,83 @@ int batch_launch(blamm_hip_batch* b)
     return BLAMM_OK;
 }
 
+// Waiting for a batch, in two steps so that a server can overlap them:
+// batch_collect waits for the kernels (replaying the batch after growing the
+// candidate or hit buffers when either overflowed) and starts the copy of the
+// grouped hits into a pinned buffer; batch_land waits for that copy.
+int batch_collect(blamm_hip_batch* b, uint64_t* nhits)
+{
+    blamm_hip_ctx* c = b->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    unsigned long long cand = 0, nh = 0;
+    for (int attempt = 0;; ++attempt) {
+        hipError_t e = hipEventSynchronize(b->ev_done);
+        if (e != hipSuccess) {
+            b->pending = false;
+            return fail(BLAMM_ERR_DEVICE, std::string("batch_wait: ") + hipGetErrorString(e));
+        }
+        cand = b->cnt_h.p[kCntCand];
+        nh = b->cnt_h.p[kCntHits];
+        if (cand <= b->cand.n && nh <= b->hits.n && nh <= b->grouped.n) break;
+        if (attempt >= 4) {
+            b->pending = false;
+            return fail(BLAMM_ERR_DEVICE, "batch_wait: capacity retries exhausted");
+        }
+        // Grow what overflowed and replay the batch (its inputs are still on
+        // the device).
+        b->replays++;
+        int rc = BLAMM_OK;
+        if (cand > b->cand.n) rc = batch_grow(b, b->cand, cand + cand / 4 + 1024);
+        if (rc == BLAMM_OK && nh > b->hits.n) rc = batch_grow(b, b->hits, nh + nh / 4 + 1024);
+        if (rc == BLAMM_OK) rc = batch_grow(b, b->grouped, b->hits.n);
+        if (rc == BLAMM_OK) rc = batch_launch(b);
+        if (rc != BLAMM_OK) {
+            b->pending = false;
+            return rc;
+        }
+    }
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev_start, b->ev_end));
+    b->stats.candidates = cand;
+    b->stats.band_rescored = b->cnt_h.p[kCntBand];
+    b->stats.hits = nh;
+    b->stats.kernel_ms = ms;
+    b->stats.filter_ms = 0.f;
+    // hits arrive grouped by request with request-local positions
+    b->copying.reset();
+    int rc = acquire_hits(b->pool, std::max<uint64_t>(nh, 1), &b->copying);
+    if (rc == BLAMM_OK && nh > 0) {
+        hipError_t e = hipMemcpyAsync(b->copying->p, b->grouped.p, nh * sizeof(GroupedHit), hipMemcpyDeviceToHost,
+                                      b->stream);
+        if (e == hipSuccess) e = hipEventRecord(b->ev_copy, b->stream);
+        if (e != hipSuccess) rc = fail(BLAMM_ERR_DEVICE, std::string("batch hit copy: ") + hipGetErrorString(e));
+    }
+    if (rc != BLAMM_OK) {
+        b->pending = false;
+        return rc;
+    }
+    b->copy_nh = nh;
+    if (nhits) *nhits = nh;
+    return BLAMM_OK;
+}
+
+int batch_land(blamm_hip_batch* b)
+{
+    if (b->copy_nh > 0) {
+        const hipError_t e = hipEventSynchronize(b->ev_copy);
+        if (e != hipSuccess) {
+            b->pending = false;
+            return fail(BLAMM_ERR_DEVICE, std::string("batch hit copy: ") + hipGetErrorString(e));
+        }
+    }
+    b->out = std::move(b->copying);
+    b->have_expanded = false;
+    b->out_off.assign(b->off_h.p, b->off_h.p + b->nreq + 1);
+    b->pending = false;
+    b->done = true;
+    return BLAMM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -263,8 +420,8 @@ int blamm_hip_batch_create(blamm_hip_ctx* c, uint64_t max_bases, uint32_t max_re
     if (e == hipSuccess) e = hipEventCreate(&b->ev_start);
     if (e == hipSuccess) e = hipEventCreate(&b->ev_end);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_copy, hipEventDisableTiming);
     if (e == hipSuccess) e = b->counters.reserve(kCntSlots);
-    if (e == hipSuccess) e = b->desc.reserve(1);
     if (e == hipSuccess) e = b->cnt_h.reserve(kCntSlots);
     const uint64_t nb = std::min<uint64_t>(max_bases, kBatchMaxBases);
     const uint64_t nr = std::max<uint32_t>(max_requests, 1);
@@ -287,18 +444,19 @@ int blamm_hip_batch_destroy(blamm_hip_batch* b)
     (void)hipSetDevice(b->ctx->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    b->packed.release(); b->lcmask.release(); b->runs.release(); b->thr.release(); b->slot_map.release();
-    b->blk_req.release(); b->groups.release(); b->frag_a.release(); b->frag_b.release(); b->band.release();
+    b->packed.release(); b->lcmask.release(); b->run_bits.release(); b->meta.release();
+    b->frag_a.release(); b->frag_b.release(); b->band.release();
     b->band_hi.release();
-    b->req_start.release();
-    b->req_end.release(); b->req_count.release(); b->req_off.release(); b->cursor.release();
-    b->items.release(); b->cand.release(); b->chunks.release(); b->desc.release(); b->counters.release();
+    b->req_count.release(); b->req_off.release(); b->cursor.release();
+    b->cand.release(); b->counters.release();
     b->hits.release(); b->grouped.release();
     b->stage.release(); b->cnt_h.release(); b->off_h.release();
     b->out.reset();
+    b->copying.reset();
     if (b->ev_start) (void)hipEventDestroy(b->ev_start);
     if (b->ev_end) (void)hipEventDestroy(b->ev_end);
     if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+    if (b->ev_copy) (void)hipEventDestroy(b->ev_copy);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return BLAMM_OK;
@@ -307,11 +465,9 @@ int blamm_hip_batch_destroy(blamm_hip_batch* b)
 uint64_t blamm_hip_batch_device_bytes(const blamm_hip_batch* b)
 {
     if (!b) return 0;
-    return b->packed.n * 8 + b->lcmask.n * 8 + b->runs.n * 8 + b->thr.n * 4 + b->slot_map.n * 4 + b->blk_req.n * 4 +
-           b->groups.n * 4 + (b->frag_a.n + b->frag_b.n) * 16 + (b->band.n + b->band_hi.n) * 8 +
-           (b->req_start.n + b->req_end.n + b->req_count.n) * 4 + (b->req_off.n + b->cursor.n) * 8 + b->items.n * 8 +
-           b->cand.n * 16 + b->chunks.n * 16 + (b->hits.n + b->grouped.n) * sizeof(blamm_hit) + sizeof(BatchDesc) +
-           kCntSlots * 8;
+    return (b->packed.n + b->lcmask.n + b->run_bits.n) * 8 + b->meta.n + (b->frag_a.n + b->frag_b.n) * 16 +
+           (b->band.n + b->band_hi.n) * 8 + b->req_count.n * 4 + (b->req_off.n + b->cursor.n) * 8 + b->cand.n * 16 +
+           b->hits.n * sizeof(DevHit) + b->grouped.n * sizeof(GroupedHit) + kCntSlots * 8;
 }
 
 int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32_t nreq)
@@ -440,20 +596,24 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     // Device capacity (growth invalidates captured graphs).
     BLAMM_TRY(batch_grow(b, b->packed, pw));
     if (mask) BLAMM_TRY(batch_grow(b, b->lcmask, mw));
-    BLAMM_TRY(batch_grow(b, b->runs, nruns + 1));
+    BLAMM_TRY(batch_grow(b, b->run_bits, total / 64 + 2));
     const size_t nbs = std::max<size_t>(smap.size(), 1);
-    BLAMM_TRY(batch_grow(b, b->thr, nbs));
-    BLAMM_TRY(batch_grow(b, b->slot_map, nbs));
+    size_t need[kMetaTables];
+    need[kMetaDesc] = 1;
+    need[kMetaReqStart] = need[kMetaReqEnd] = nreq;
+    need[kMetaThr] = need[kMetaSlotMap] = nbs;
+    need[kMetaBlkReq] = std::max<size_t>(nblk, 1);
+    need[kMetaGroups] = std::max<size_t>(gpk.size(), 1);
+    need[kMetaChunks] = std::max<size_t>(chunks.size(), 1);
+    need[kMetaRuns] = nruns + 1;
+    need[kMetaItems] = std::max<size_t>(nitems, 1);
+    BLAMM_TRY(meta_fit(b, need));
     BLAMM_TRY(batch_grow(b, b->band, nbs));
     BLAMM_TRY(batch_grow(b, b->band_hi, nbs));
-    BLAMM_TRY(batch_grow(b, b->blk_req, std::max<size_t>(nblk, 1)));
-    BLAMM_TRY(batch_grow(b, b->groups, std::max<size_t>(gpk.size(), 1)));
     const size_t fr = (size_t)std::max<uint32_t>(nblk, 1) * (bf ? (size_t)kMaxChunks * 64 : (size_t)kMaxLen * 16);
     BLAMM_TRY(batch_grow(b, b->frag_a, fr));
     if (bf) BLAMM_TRY(batch_grow(b, b->frag_b, fr));
     else BLAMM_TRY(batch_grow(b, b->frag_b, 1));
-    BLAMM_TRY(batch_grow(b, b->req_start, nreq));
-    BLAMM_TRY(batch_grow(b, b->req_end, nreq));
     BLAMM_TRY(batch_grow(b, b->req_count, nreq));
     BLAMM_TRY(batch_grow(b, b->req_off, (size_t)nreq + 1));
     BLAMM_TRY(batch_grow(b, b->cursor, nreq));
@@ -461,7 +621,6 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
         HIP_TRY(hipStreamSynchronize(b->stream));
         HIP_TRY(b->off_h.reserve((size_t)nreq + 1 + nreq / 8));
     }
-    BLAMM_TRY(batch_grow(b, b->items, std::max<size_t>(nitems, 1)));
     // room for the expected candidates and hits (records <= hits + rescored
     // windows; 1.5x margin), so the batch rarely needs a second run
     uint64_t expect = 0;
@@ -472,32 +631,25 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
         BLAMM_TRY(batch_grow(b, b->hits, expect));
         BLAMM_TRY(batch_grow(b, b->grouped, b->hits.n));
     }
-    BLAMM_TRY(batch_grow(b, b->chunks, std::max<size_t>(chunks.size(), 1)));
 
-    // Stage everything in pinned memory, then one copy per buffer.
+    // Stage everything in pinned memory: the packed stream, the lowercase
+    // mask, then the tables in the arena's layout; three copies in all.
     const size_t o_pk = 0;
     const size_t o_mk = align_up(o_pk + pw * 8, 256);
-    const size_t o_rn = align_up(o_mk + mw * 8, 256);
-    const size_t o_tb = align_up(o_rn + (nruns + 1) * 8, 256);        // thresholds
-    const size_t o_sm = align_up(o_tb + sthr.size() * 4, 256);        // slot map
-    const size_t o_br = align_up(o_sm + smap.size() * 4, 256);        // block requests
-    const size_t o_gp = align_up(o_br + (size_t)nblk * 4, 256);       // groups
-    const size_t o_re = align_up(o_gp + gpk.size() * 4, 256);
-    const size_t o_rs = align_up(o_re + (size_t)nreq * 4, 256);
-    const size_t o_ch = align_up(o_rs + (size_t)nreq * 4, 256);
-    const size_t o_it = align_up(o_ch + chunks.size() * 16, 256);
-    const size_t o_ds = align_up(o_it + nitems * 8, 256);
-    const size_t o_end = o_ds + sizeof(BatchDesc);
+    const size_t o_mt = align_up(o_mk + mw * 8, 256);
+    const size_t o_end = o_mt + b->meta_off[kMetaTables];
     if (o_end > b->stage.n) {
         HIP_TRY(hipStreamSynchronize(b->stream));
-        HIP_TRY(b->stage.reserve(o_end + o_end / 8));
+        HIP_TRY(b->stage.reserve(o_end + o_end / 2));
     }
     uint8_t* S = b->stage.p;
+    uint8_t* M = S + o_mt;
+    auto table = [&](int x) { return M + b->meta_off[x]; };
     uint64_t* pk = (uint64_t*)(S + o_pk);
     uint64_t* mk = (uint64_t*)(S + o_mk);
-    uint64_t* rn = (uint64_t*)(S + o_rn);
-    uint32_t* re = (uint32_t*)(S + o_re);
-    uint32_t* rs = (uint32_t*)(S + o_rs);
+    uint64_t* rn = (uint64_t*)table(kMetaRuns);
+    uint32_t* re = (uint32_t*)table(kMetaReqEnd);
+    uint32_t* rs = (uint32_t*)table(kMetaReqStart);
     std::memset(pk, 0, pw * 8);
     if (mask) std::memset(mk, 0, mw * 8);
     size_t ir = 0;
@@ -515,33 +667,23 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
         rs[r] = (uint32_t)s0;
     }
     d.nruns = (uint32_t)ir;
-    std::memcpy(S + o_tb, sthr.data(), sthr.size() * 4);
-    std::memcpy(S + o_sm, smap.data(), smap.size() * 4);
-    std::memcpy(S + o_br, breq.data(), (size_t)nblk * 4);
-    std::memcpy(S + o_gp, gpk.data(), gpk.size() * 4);
-    std::memcpy(S + o_ch, chunks.data(), chunks.size() * 16);
+    std::memcpy(table(kMetaThr), sthr.data(), sthr.size() * 4);
+    std::memcpy(table(kMetaSlotMap), smap.data(), smap.size() * 4);
+    std::memcpy(table(kMetaBlkReq), breq.data(), (size_t)nblk * 4);
+    std::memcpy(table(kMetaGroups), gpk.data(), gpk.size() * 4);
+    std::memcpy(table(kMetaChunks), chunks.data(), chunks.size() * 16);
     size_t at = 0;
     for (int x = 0; x < kXcds; ++x) {
-        std::memcpy(S + o_it + at * 8, sub[x].data(), sub[x].size() * 8);
+        std::memcpy(table(kMetaItems) + at * 8, sub[x].data(), sub[x].size() * 8);
         at += sub[x].size();
     }
-    std::memcpy(S + o_ds, &d, sizeof(d));
+    std::memcpy(table(kMetaDesc), &d, sizeof(d));
 
     hipStream_t s = b->stream;
     HIP_TRY(hipMemcpyAsync(b->packed.p, S + o_pk, pw * 8, hipMemcpyHostToDevice, s));
     if (mask) HIP_TRY(hipMemcpyAsync(b->lcmask.p, S + o_mk, mw * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(b->runs.p, S + o_rn, std::max<size_t>(ir, 1) * 8, hipMemcpyHostToDevice, s));
-    if (!smap.empty()) {
-        HIP_TRY(hipMemcpyAsync(b->thr.p, S + o_tb, sthr.size() * 4, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(b->slot_map.p, S + o_sm, smap.size() * 4, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(b->blk_req.p, S + o_br, (size_t)nblk * 4, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(b->groups.p, S + o_gp, gpk.size() * 4, hipMemcpyHostToDevice, s));
-    }
-    HIP_TRY(hipMemcpyAsync(b->req_end.p, S + o_re, (size_t)nreq * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(b->req_start.p, S + o_rs, (size_t)nreq * 4, hipMemcpyHostToDevice, s));
-    if (!chunks.empty()) HIP_TRY(hipMemcpyAsync(b->chunks.p, S + o_ch, chunks.size() * 16, hipMemcpyHostToDevice, s));
-    if (nitems) HIP_TRY(hipMemcpyAsync(b->items.p, S + o_it, nitems * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(b->desc.p, S + o_ds, sizeof(BatchDesc), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->meta.p, M, b->meta_off[kMetaItems] + std::max<size_t>(nitems, 1) * 8,
+                           hipMemcpyHostToDevice, s));
 
     b->mask = mask;
     b->nreq = nreq;
@@ -565,52 +707,8 @@ int blamm_hip_batch_wait(blamm_hip_batch* b, uint64_t* nhits)
 {
     if (!b) return fail(BLAMM_ERR_ARG, "batch_wait: null batch");
     if (!b->pending) return fail(BLAMM_ERR_STATE, "batch_wait: no batch in flight");
-    blamm_hip_ctx* c = b->ctx;
-    HIP_TRY(hipSetDevice(c->device));
-    unsigned long long cand = 0, nh = 0;
-    for (int attempt = 0;; ++attempt) {
-        hipError_t e = hipEventSynchronize(b->ev_done);
-        if (e != hipSuccess) {
-            b->pending = false;
-            return fail(BLAMM_ERR_DEVICE, std::string("batch_wait: ") + hipGetErrorString(e));
-        }
-        cand = b->cnt_h.p[kCntCand];
-        nh = b->cnt_h.p[kCntHits];
-        if (cand <= b->cand.n && nh <= b->hits.n) break;
-        if (attempt >= 4) {
-            b->pending = false;
-            return fail(BLAMM_ERR_DEVICE, "batch_wait: capacity retries exhausted");
-        }
-        // Grow what overflowed and replay the batch (its inputs are still on
-        // the device).
-        b->replays++;
-        if (cand > b->cand.n) BLAMM_TRY(batch_grow(b, b->cand, cand + cand / 4 + 1024));
-        if (nh > b->hits.n) {
-            BLAMM_TRY(batch_grow(b, b->hits, nh + nh / 4 + 1024));
-            BLAMM_TRY(batch_grow(b, b->grouped, b->hits.n));
-        }
-        BLAMM_TRY(batch_launch(b));
-    }
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, b->ev_start, b->ev_end));
-    b->stats.candidates = cand;
-    b->stats.band_rescored = b->cnt_h.p[kCntBand];
-    b->stats.hits = nh;
-    b->stats.kernel_ms = ms;
-    b->stats.filter_ms = 0.f;
-    // hits arrive grouped by request with request-local positions
-    std::shared_ptr<HitBuf> buf;
-    BLAMM_TRY(acquire_hits(b->pool, std::max<uint64_t>(nh, 1), &buf));
-    if (nh > 0) {
-        HIP_TRY(hipMemcpyAsync(buf->p, b->grouped.p, nh * sizeof(blamm_hit), hipMemcpyDeviceToHost, b->stream));
-        HIP_TRY(hipStreamSynchronize(b->stream));
-    }
-    b->out = buf;
-    b->out_off.assign(b->off_h.p, b->off_h.p + b->nreq + 1);
-    b->pending = false;
-    b->done = true;
-    if (nhits) *nhits = nh;
-    return BLAMM_OK;
+    BLAMM_TRY(batch_collect(b, nhits));
+    return batch_land(b);
 }
 
 int blamm_hip_batch_hits(blamm_hip_batch* b, uint32_t req, const blamm_hit** hits, uint64_t* n)
@@ -618,7 +716,13 @@ int blamm_hip_batch_hits(blamm_hip_batch* b, uint32_t req, const blamm_hit** hit
     if (!b || !hits || !n) return fail(BLAMM_ERR_ARG, "batch_hits: null pointer");
     if (!b->done) return fail(BLAMM_ERR_STATE, "batch_hits: no finished batch");
     if (req >= b->nreq) return fail(BLAMM_ERR_ARG, "batch_hits: request index out of range");
-    *hits = b->out->p + b->out_off[req];
+    if (!b->have_expanded) {
+        const uint64_t nh = b->out_off[b->nreq];
+        b->expanded.resize(std::max<uint64_t>(nh, 1));
+        expand_hits(b->out->p, nh, b->expanded.data());
+        b->have_expanded = true;
+    }
+    *hits = b->expanded.data() + b->out_off[req];
     *n = b->out_off[req + 1] - b->out_off[req];
     return BLAMM_OK;
 }

@@ -196,6 +196,7 @@ struct blamm_hip_ctx {
     // scratch
     DevBuf<uint4> cand;
     DevBuf<unsigned long long> counters;
+    DevBuf<uint64_t> run_bits;        // run-start bits of the stream being scanned
 
     // host-buffer staging
     DevBuf<uint64_t> packed, lcmask, runs, nhits_dev;
@@ -534,6 +535,7 @@ FinalizeArgs finalize_args(const blamm_hip_ctx* c, const uint4* cand, uint64_t c
     f.nbases = 0;
     f.run_starts = runs;
     f.nruns = 0;
+    f.run_bits = nullptr;
     f.slot_len = c->slot_len.p;
     f.slot_w = c->slot_w.p;
     f.slot_thr = c->slot_thr.p;
@@ -545,6 +547,7 @@ FinalizeArgs finalize_args(const blamm_hip_ctx* c, const uint4* cand, uint64_t c
     f.req_end = nullptr;
     f.slot_map = nullptr;
     f.blk_req = nullptr;
+    f.req_count = nullptr;
     return f;
 }
 
@@ -617,7 +620,7 @@ int blamm_hip_destroy(blamm_hip_ctx* c)
     c->slot_band_any.release(); c->slot_band_hi.release(); c->slot_band_hi_any.release();
     c->slot_seq.release(); c->slot_w.release();
     c->slot_len.release(); c->slot_col.release(); c->groups_d.release(); c->cand.release();
-    c->counters.release();
+    c->counters.release(); c->run_bits.release();
     c->packed.release(); c->lcmask.release(); c->runs.release(); c->nhits_dev.release();
     c->hits.release();
     c->hits_pinned.release();
@@ -733,14 +736,26 @@ int check_scan_args(const blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
 
 // Enqueue counter clear, filter, finalize (and the hit-count copy) on s, with
 // the filter records in cand[0..cand_cap) and the counters in `counters`.
+// With run_bits (nbases / 64 + 2 words) finalize checks the run rule from
+// run-start bits built here; without, it searches the run starts.
 int enqueue_scan(blamm_hip_ctx* c, const uint64_t* d_packed, const uint64_t* d_lcmask, uint64_t nbases,
                  const uint64_t* d_run_starts, uint64_t nruns, blamm_hit* d_hits, uint64_t cap, uint64_t* d_nhits,
-                 uint4* cand, uint64_t cand_cap, unsigned long long* counters, hipStream_t s)
+                 uint4* cand, uint64_t cand_cap, unsigned long long* counters, uint64_t* run_bits, hipStream_t s)
 {
     const bool bf = prec_bf(c->precision);
     const bool mask = d_lcmask != nullptr;
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counters);
     HIP_TRY(hipGetLastError());
+    if (nruns == 0) run_bits = nullptr;   // nothing to search
+    if (run_bits) {
+        const uint32_t words = (uint32_t)(nbases / 64 + 2);
+        hipLaunchKernelGGL(run_bits_clear_kernel, dim3(std::min<uint32_t>((words + 255) / 256, 4096)), dim3(256), 0, s,
+                           run_bits, (uint32_t)nbases, (const BatchDesc*)nullptr);
+        hipLaunchKernelGGL(run_bits_set_kernel, dim3(std::min<uint32_t>((uint32_t)((nruns + 255) / 256), 4096)),
+                           dim3(256), 0, s, run_bits, d_run_starts, (uint32_t)nruns, (uint32_t)nbases,
+                           (const BatchDesc*)nullptr);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipEventRecord(c->ev[0], s));
     if (nbases > 0) {
         const uint32_t ntiles = (uint32_t)((nbases + kBf16Tile - 1) / kBf16Tile);   // 32-window tiles
@@ -764,6 +779,7 @@ int enqueue_scan(blamm_hip_ctx* c, const uint64_t* d_packed, const uint64_t* d_l
         FinalizeArgs f = finalize_args(c, cand, cand_cap, counters, d_packed, d_lcmask, d_run_starts);
         f.nbases = (uint32_t)nbases;
         f.nruns = (uint32_t)nruns;
+        f.run_bits = run_bits;
         f.hits = d_hits;
         f.hit_cap = cap;
         hipError_t e = launch_finalize(bf, false, f, (uint32_t)c->num_cu * 8, s);
@@ -796,9 +812,10 @@ int blamm_hip_scan_device(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t p
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     if (c->cand.n == 0) HIP_TRY(c->cand.reserve(1u << 20));
+    if (nruns > 0 && c->run_bits.n < nbases / 64 + 2) HIP_TRY(c->run_bits.reserve(nbases / 64 + 2));
     c->pending_noalloc = false;
     return enqueue_scan(c, d_packed, d_lcmask, nbases, d_run_starts, nruns, d_hits, cap, d_nhits, c->cand.p,
-                        c->cand.n, c->counters.p, s);
+                        c->cand.n, c->counters.p, c->run_bits.p, s);
 }
 
 int blamm_hip_scan_device_noalloc(blamm_hip_ctx* c, const uint64_t* d_packed, uint64_t packed_words,
@@ -821,7 +838,7 @@ int blamm_hip_scan_device_noalloc(blamm_hip_ctx* c, const uint64_t* d_packed, ui
     c->pending_noalloc = true;
     return enqueue_scan(c, d_packed, d_lcmask, nbases, d_run_starts, nruns, d_hits, cap, d_nhits,
                         (uint4*)(base + kNoAllocHeader), (uint64_t)BLAMM_NOALLOC_MAX_RECORDS,
-                        (unsigned long long*)base, s);
+                        (unsigned long long*)base, nullptr, s);
 }
 
 int blamm_hip_scan_finish(blamm_hip_ctx* c, uint64_t* nhits)

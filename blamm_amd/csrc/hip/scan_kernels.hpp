@@ -616,6 +616,15 @@ struct WorkQueue {
 // Batch descriptor in device memory (list form).  The host writes it before
 // each launch, so one captured graph serves every batch that fits the
 // buffers it was captured with.
+// A batch hit as the host receives it: position within its request, the
+// caller's column and the score, 12 bytes (a quarter less read-back than
+// the 16-byte blamm_hit, which the host rebuilds on demand).
+struct GroupedHit {
+    uint32_t pos;
+    uint32_t col;
+    float score;
+};
+
 struct BatchDesc {
     uint32_t sub_begin[kXcds + 1];
     uint32_t nreq;
@@ -739,6 +748,7 @@ struct FinalizeArgs {
     uint32_t nbases;
     const uint64_t* run_starts;
     uint32_t nruns;
+    const uint64_t* run_bits;             // bit i set: a run starts at base i (null: search run_starts)
     const int* slot_len;
     const float* slot_w;                  // [slot][kMaxLen][4] FP32 weights
     const float* slot_thr;                // the threshold per slot (list form: per batch slot)
@@ -750,6 +760,7 @@ struct FinalizeArgs {
     const uint32_t* req_end;              // list form: end of each request's bases
     const uint32_t* slot_map;             // list form: batch slot -> library slot
     const uint32_t* blk_req;              // list form: request of each batch block
+    uint32_t* req_count;                  // list form: hits per request (cleared before the launch)
 };
 
 struct DevHit {
@@ -757,6 +768,38 @@ struct DevHit {
     uint32_t col;
     float score;
 };
+
+// Requests counted in LDS by finalize and the grouping kernels; batches with
+// more requests count with global atomics.
+constexpr int kGroupLds = 1024;
+constexpr int kGroupThreads = 256;
+
+// For the lanes with `on` set, add one per lane to counter[key] with one
+// atomic per distinct key in the wave, and return the counter's value before
+// the wave's addition plus the lane's rank among the lanes sharing its key
+// (so lanes of one key get consecutive values).  The hits of one wave come
+// from the records of one filter item, so there is usually a single key.
+template <typename T>
+__device__ __forceinline__ T wave_add_by_key(T* counter, uint32_t key, bool on)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t todo = __ballot(on);
+    T mine = 0;
+    while (todo) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
+        const uint32_t k = (uint32_t)__shfl((int)key, (int)leader);
+        const bool mem = on && key == k;
+        const uint64_t same = __ballot(mem);
+        T base = 0;
+        if (lane == leader) base = atomicAdd(&counter[k], (T)__popcll(same));
+        base = __shfl(base, (int)leader);
+        if (mem)
+            mine = base + (T)__builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+        todo &= ~same;
+    }
+    return mine;
+}
 
 // Hits staged per wave before one atomic reserves their range: every flush
 // is an atomic on one counter, and at ~30M hits per scan those atomics bound
@@ -798,6 +841,17 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
     __shared__ DevHit hit_lds[kWavesPerBlock][kHitStage];
     DevHit* sbuf = hit_lds[threadIdx.x >> 6];
     uint32_t sn = 0;
+    // list form: hits per request, for the grouping that follows
+    __shared__ uint32_t req_lds[LIST ? kGroupLds : 1];
+    uint32_t nreq = 0;
+    bool req_in_lds = false;
+    if constexpr (LIST) {
+        nreq = __builtin_amdgcn_readfirstlane(a.desc->nreq);
+        req_in_lds = nreq <= (uint32_t)kGroupLds;
+        if (req_in_lds)
+            for (uint32_t i = threadIdx.x; i < nreq; i += kThreads) req_lds[i] = 0u;
+        __syncthreads();
+    }
     auto flush = [&]() {
         if (sn == 0) return;
         unsigned long long at0 = 0;
@@ -830,7 +884,7 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
         const int L = have ? a.slot_len[lib] : 0;
         const float* w = a.slot_w + (size_t)lib * kMaxLen * 4;
         uint32_t nb1 = 0;
-        if (have) {
+        if (have && !a.run_bits) {
             // first run start after q0
             uint32_t l = 0, h = nruns;
             while (l < h) {
@@ -876,17 +930,26 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             bool hit = false;
             float score = s;
             if (act && q < qend && s >= slo) {
-                // run-boundary rule: next run start after q
-                uint32_t nb = nb1;
-                if (q >= nb1) {
-                    uint32_t l = 0, h = nruns;
-                    while (l < h) {
-                        const uint32_t mid = (l + h) >> 1;
-                        if (a.run_starts[mid] <= q) l = mid + 1; else h = mid;
+                // run-boundary rule: no run starts at q + 1 .. q + L - 1 and
+                // the window ends inside the stream (and its request)
+                bool inside = (uint64_t)q + (uint64_t)L <= (uint64_t)min(nbases, qend);
+                if (a.run_bits) {
+                    const uint64_t starts = load_bits64(a.run_bits, (uint64_t)q + 1);
+                    inside = inside && (starts & ((1ull << (L - 1)) - 1ull)) == 0;
+                } else {
+                    // next run start after q
+                    uint32_t nb = nb1;
+                    if (q >= nb1) {
+                        uint32_t l = 0, h = nruns;
+                        while (l < h) {
+                            const uint32_t mid = (l + h) >> 1;
+                            if (a.run_starts[mid] <= q) l = mid + 1; else h = mid;
+                        }
+                        nb = l < nruns ? (uint32_t)a.run_starts[l] : nbases;
                     }
-                    nb = l < nruns ? (uint32_t)a.run_starts[l] : nbases;
+                    inside = inside && (uint64_t)q + (uint64_t)L <= (uint64_t)nb;
                 }
-                if ((uint64_t)q + (uint64_t)L <= (uint64_t)min(nb, qend)) {
+                if (inside) {
                     if (s >= shi) {
                         hit = true;
                     } else {
@@ -907,6 +970,10 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
             }
             const uint64_t m = __ballot(hit);
             if (m) {
+                if constexpr (LIST) {
+                    if (req_in_lds) (void)wave_add_by_key(req_lds, req, hit);
+                    else (void)wave_add_by_key(a.req_count, req, hit);
+                }
                 if (sn > (uint32_t)(kHitStage - 64)) flush();
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -922,6 +989,13 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
         }
     }
     flush();
+    if constexpr (LIST) {
+        if (req_in_lds) {
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < nreq; i += kThreads)
+                if (req_lds[i]) atomicAdd(&a.req_count[i], req_lds[i]);
+        }
+    }
 }
 
 #ifndef BLAMM_INSTANTIATE   // small kernels: launched from the main unit only
@@ -929,25 +1003,50 @@ static __global__ void zero_counters_kernel(unsigned long long* c)
 {
     if (threadIdx.x < kCntSlots) c[threadIdx.x] = 0ull;
 }
+
+// Run-start bits for finalize_kernel: bit i is set when a run starts at base
+// i (0 < i < nbases).  The clear kernel zeroes words [0, nbases / 64 + 2)
+// (what load_bits64 may read below q + 1 <= nbases); the set kernel then
+// sets one bit per run start.  List form: sizes from the batch descriptor.
+static __global__ __launch_bounds__(256) void run_bits_clear_kernel(uint64_t* bits, uint32_t nbases,
+                                                                    const BatchDesc* desc)
+{
+    const uint32_t n = desc ? desc->nbases : nbases;
+    const uint64_t words = (uint64_t)n / 64 + 2;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (uint64_t)gridDim.x * 256)
+        bits[i] = 0ull;
+}
+
+static __global__ __launch_bounds__(256) void run_bits_set_kernel(uint64_t* bits, const uint64_t* starts,
+                                                                  uint32_t nruns, uint32_t nbases,
+                                                                  const BatchDesc* desc)
+{
+    const uint32_t n = desc ? desc->nbases : nbases;
+    const uint32_t nr = desc ? desc->nruns : nruns;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nr; i += gridDim.x * 256) {
+        const uint64_t st = starts[i];
+        if (st > 0 && st < n) atomicOr((unsigned long long*)&bits[st >> 6], 1ull << (st & 63));
+    }
+}
 #endif
 
 // ---------------------------------------------------------------------------
-// Hits of a batch grouped by request on the device (count, scan, scatter), so
-// the host receives each request's hits contiguously with request-local
-// positions.  Per-request counts and range reservations are taken per block
-// in LDS when the batch has at most kGroupLds requests (one global atomic per
-// block and request instead of one per hit).
+// Hits of a batch grouped by request on the device, so the host receives each
+// request's hits contiguously with request-local positions: finalize_kernel
+// counts the hits per request, group_scan_kernel turns the counts into
+// offsets, group_scatter_kernel places the hits (12-byte GroupedHit).  Range
+// reservations are taken per block and request in LDS when the batch has at
+// most kGroupLds requests (one global atomic per block and request).
 // ---------------------------------------------------------------------------
-constexpr int kGroupLds = 4096;
-constexpr int kGroupThreads = 256;
 
 struct GroupArgs {
     const BatchDesc* desc;
     const unsigned long long* counters;   // kCntHits
-    uint64_t cap;                         // hit buffer capacity
+    uint64_t cap;                         // finalize output capacity
+    uint64_t out_cap;                     // grouped output capacity
     const DevHit* in;                     // finalize output, pos = request << 32 | batch position
-    DevHit* out;                          // grouped, request-local positions
-    uint32_t* req_count;                  // [nreq], cleared by group_clear_kernel
+    GroupedHit* out;                      // grouped, request-local positions
+    uint32_t* req_count;                  // [nreq] hits per request, cleared by group_clear_kernel
     unsigned long long* req_off;          // [nreq + 1] exclusive offsets
     unsigned long long* cursor;           // [nreq] next free slot per request
     const uint32_t* req_start;            // [nreq] first batch position of each request
@@ -965,27 +1064,6 @@ __device__ __forceinline__ uint64_t group_nhits(const GroupArgs& g)
 {
     const uint64_t n = g.counters[kCntHits];
     return n < g.cap ? n : g.cap;
-}
-
-static __global__ __launch_bounds__(kGroupThreads) void group_count_kernel(GroupArgs g)
-{
-    __shared__ uint32_t cnt[kGroupLds];
-    const uint32_t nreq = g.desc->nreq;
-    const uint64_t n = group_nhits(g);
-    const bool lds = nreq <= (uint32_t)kGroupLds;
-    if (lds)
-        for (uint32_t i = threadIdx.x; i < nreq; i += kGroupThreads) cnt[i] = 0u;
-    __syncthreads();
-    for (uint64_t i = (uint64_t)blockIdx.x * kGroupThreads + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * kGroupThreads) {
-        const uint32_t r = (uint32_t)(g.in[i].pos >> 32);
-        if (lds) atomicAdd(&cnt[r], 1u);
-        else atomicAdd(&g.req_count[r], 1u);
-    }
-    __syncthreads();
-    if (lds)
-        for (uint32_t i = threadIdx.x; i < nreq; i += kGroupThreads)
-            if (cnt[i]) atomicAdd(&g.req_count[i], cnt[i]);
 }
 
 // One block: exclusive prefix over the request counts.
@@ -1014,8 +1092,12 @@ static __global__ __launch_bounds__(1024) void group_scan_kernel(GroupArgs g)
     if (threadIdx.x == 1023) g.req_off[nreq] = part[1023];
 }
 
-// Each block scatters a contiguous range of hits: counts per request in LDS,
-// reserves one global range per request, then places hits by LDS atomics.
+// Each block places a contiguous range of hits: counts per request in LDS,
+// one global range per request and block, then ranks within the range.  The
+// per-request counts include hits that finalize could not store (capacity
+// overflow: the batch is then run again), so every write is bounds-checked.
+// (Writing the grouped hits straight into pinned host memory instead of
+// device memory measured slower: the kernel then waits on PCIe.)
 static __global__ __launch_bounds__(kGroupThreads) void group_scatter_kernel(GroupArgs g)
 {
     __shared__ uint32_t cnt[kGroupLds];
@@ -1025,31 +1107,52 @@ static __global__ __launch_bounds__(kGroupThreads) void group_scatter_kernel(Gro
     const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
     const uint64_t b0 = (uint64_t)blockIdx.x * per;
     const uint64_t b1 = b0 + per < n ? b0 + per : n;
+    // the loops run while any lane of the wave has a hit left (the key
+    // aggregation is wave-wide)
+    const uint64_t wb = b0 + (threadIdx.x & ~63u);
+    auto place = [&](const DevHit& h, uint32_t r, uint64_t at) {
+        if (at < g.out_cap) {
+            GroupedHit o;
+            o.pos = (uint32_t)(h.pos & 0xFFFFFFFFull) - g.req_start[r];
+            o.col = h.col;
+            o.score = h.score;
+            g.out[at] = o;
+        }
+    };
     if (nreq > (uint32_t)kGroupLds) {
-        for (uint64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) {
-            DevHit h = g.in[i];
-            const uint32_t r = (uint32_t)(h.pos >> 32);
-            const unsigned long long at = atomicAdd(&g.cursor[r], 1ull);
-            h.pos = (h.pos & 0xFFFFFFFFull) - g.req_start[r];
-            g.out[at] = h;
+        for (uint64_t w = wb; w < b1; w += kGroupThreads) {
+            const uint64_t i = w + (threadIdx.x & 63);
+            const bool on = i < b1;
+            DevHit h{};
+            if (on) h = g.in[i];
+            const uint32_t r = on ? (uint32_t)(h.pos >> 32) : 0u;
+            const unsigned long long at = wave_add_by_key(g.cursor, r, on);
+            if (on) place(h, r, at);
         }
         return;
     }
     for (uint32_t i = threadIdx.x; i < nreq; i += kGroupThreads) cnt[i] = 0u;
     __syncthreads();
-    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) atomicAdd(&cnt[(uint32_t)(g.in[i].pos >> 32)], 1u);
+    for (uint64_t w = wb; w < b1; w += kGroupThreads) {
+        const uint64_t i = w + (threadIdx.x & 63);
+        const bool on = i < b1;
+        const uint32_t r = on ? (uint32_t)(g.in[i].pos >> 32) : 0u;
+        (void)wave_add_by_key(cnt, r, on);
+    }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nreq; i += kGroupThreads) {
         base[i] = cnt[i] ? atomicAdd(&g.cursor[i], (unsigned long long)cnt[i]) : 0ull;
         cnt[i] = 0u;
     }
     __syncthreads();
-    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kGroupThreads) {
-        DevHit h = g.in[i];
-        const uint32_t r = (uint32_t)(h.pos >> 32);
-        const unsigned long long at = base[r] + atomicAdd(&cnt[r], 1u);
-        h.pos = (h.pos & 0xFFFFFFFFull) - g.req_start[r];
-        g.out[at] = h;
+    for (uint64_t w = wb; w < b1; w += kGroupThreads) {
+        const uint64_t i = w + (threadIdx.x & 63);
+        const bool on = i < b1;
+        DevHit h{};
+        if (on) h = g.in[i];
+        const uint32_t r = on ? (uint32_t)(h.pos >> 32) : 0u;
+        const uint32_t k = wave_add_by_key(cnt, r, on);
+        if (on) place(h, r, base[r] + k);
     }
 }
 #endif  // BLAMM_INSTANTIATE

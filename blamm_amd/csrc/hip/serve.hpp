@@ -1,14 +1,16 @@
 // Serving loop: a resident library, requests admitted by an HBM budget and
 // batched as they arrive (continuous batching).  The server owns `slots`
-// batch slots and two threads:
+// batch slots and three threads, one per stage of a slot:
 //  - the launcher takes a free slot whenever one exists, packs the queued
 //    requests (first come, first served) that fit the slot's share of the
 //    budget into one batch and launches it while other slots' batches are
 //    still on the device;
-//  - the harvester waits for the oldest batch in flight, copies its hits
-//    (grouped by request) into pinned memory and hands them to the requests,
-//    then frees the slot.
-// So a batch's hit read-back never holds up the launch of the next one.
+//  - the harvester waits for the oldest batch in flight and starts the copy
+//    of its hits (grouped by request) into pinned memory;
+//  - the completer waits for the oldest copy, hands the hits to the
+//    requests and frees the slot.
+// So a batch's hit read-back overlaps the next batch's kernels and never
+// holds up a launch.
 // Every request's result is the same as if it had been scanned alone:
 // thresholds, run rule and scores are per request and per window.
 //
@@ -44,9 +46,10 @@ struct ServeRequest {
 struct ServeSlot {
     blamm_hip_batch* batch = nullptr;
     std::vector<uint64_t> ids;       // requests of the batch in flight
-    // free -> launching (launcher) -> in flight -> harvesting (harvester) -> free
-    enum State { kFree, kLaunching, kInFlight, kHarvesting } state = kFree;
-    uint64_t launched = 0;           // launch order, for harvesting oldest first
+    // free -> launching (launcher) -> in flight -> harvesting (harvester) ->
+    // copying -> completing (completer) -> free
+    enum State { kFree, kLaunching, kInFlight, kHarvesting, kCopying, kCompleting } state = kFree;
+    uint64_t launched = 0;           // launch order, for harvesting and completing oldest first
 };
 
 }  // namespace
@@ -58,13 +61,14 @@ struct blamm_hip_server {
     std::vector<ServeSlot> slots;
     std::mutex mu;
     // cv_work: a request was queued or a slot was freed (launcher);
-    // cv_flight: a batch was launched (harvester); cv_done: requests finished
-    std::condition_variable cv_work, cv_flight, cv_done;
+    // cv_flight: a batch was launched (harvester); cv_copy: a hit copy was
+    // started (completer); cv_done: requests finished
+    std::condition_variable cv_work, cv_flight, cv_copy, cv_done;
     std::deque<uint64_t> queue;
     std::unordered_map<uint64_t, ServeRequest> reqs;
     uint64_t next_id = 1, launches = 0;
     bool stop = false;
-    std::thread launcher, harvester;
+    std::thread launcher, harvester, completer;
     blamm_server_stats stats{};
 };
 
@@ -172,8 +176,8 @@ void serve_launch_loop(blamm_hip_server* S)
     S->cv_flight.notify_all();
 }
 
-// Harvester thread: finish batches oldest first (wait, hit read-back,
-// hand-out), freeing their slots.
+// Harvester thread: wait for batches oldest first and start their hit
+// copies.
 void serve_harvest_loop(blamm_hip_server* S)
 {
     (void)hipSetDevice(S->ctx->device);
@@ -195,7 +199,7 @@ void serve_harvest_loop(blamm_hip_server* S)
         uint64_t nh = 0;
         const uint64_t t0 = now_ns();
         const uint64_t rp0 = oldest->batch->replays;
-        const int rc = blamm_hip_batch_wait(oldest->batch, &nh);
+        const int rc = batch_collect(oldest->batch, &nh);
         blamm_scan_stats bs{};
         blamm_hip_batch_stats(oldest->batch, &bs);
         const uint64_t t1 = now_ns();
@@ -203,6 +207,42 @@ void serve_harvest_loop(blamm_hip_server* S)
         S->stats.device_ms += bs.kernel_ms;
         S->stats.host_wait_ms += (t1 - t0) / 1e6;
         S->stats.replays += oldest->batch->replays - rp0;
+        if (rc != BLAMM_OK) {
+            serve_finish(S, *oldest, rc, lk);
+        } else {
+            oldest->state = ServeSlot::kCopying;
+            S->cv_copy.notify_all();
+        }
+    }
+    S->cv_copy.notify_all();
+}
+
+// Completer thread: wait for hit copies oldest first, hand the hits to the
+// requests and free the slots.
+void serve_complete_loop(blamm_hip_server* S)
+{
+    (void)hipSetDevice(S->ctx->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        ServeSlot* oldest = nullptr;
+        S->cv_copy.wait(lk, [&] {
+            oldest = nullptr;
+            bool busy = false;
+            for (ServeSlot& x : S->slots) {
+                if (x.state == ServeSlot::kCopying && (!oldest || x.launched < oldest->launched)) oldest = &x;
+                busy = busy || x.state == ServeSlot::kLaunching || x.state == ServeSlot::kInFlight ||
+                       x.state == ServeSlot::kHarvesting;
+            }
+            return oldest != nullptr || (S->stop && S->queue.empty() && !busy);
+        });
+        if (!oldest) break;
+        oldest->state = ServeSlot::kCompleting;
+        lk.unlock();
+        const uint64_t t0 = now_ns();
+        const int rc = batch_land(oldest->batch);
+        const uint64_t t1 = now_ns();
+        lk.lock();
+        S->stats.host_wait_ms += (t1 - t0) / 1e6;
         serve_finish(S, *oldest, rc, lk);
     }
 }
@@ -220,7 +260,7 @@ int blamm_hip_server_create(blamm_hip_ctx* c, const blamm_server_config* cfg, bl
     if (cfg) k = *cfg;
     if (k.hbm_budget_bytes == 0) k.hbm_budget_bytes = 8ull << 30;
     if (k.slots == 0) k.slots = 2;
-    if (k.slots > 4) return fail(BLAMM_ERR_ARG, "server_create: at most 4 slots");
+    if (k.slots > 8) return fail(BLAMM_ERR_ARG, "server_create: at most 8 slots");
     if (k.max_batch_requests == 0) k.max_batch_requests = 256;
     if (k.max_batch_bases == 0) k.max_batch_bases = 1ull << 24;
     k.max_batch_bases = std::min<uint64_t>(k.max_batch_bases, kBatchMaxBases);
@@ -241,6 +281,7 @@ int blamm_hip_server_create(blamm_hip_ctx* c, const blamm_server_config* cfg, bl
     }
     S->launcher = std::thread(serve_launch_loop, S);
     S->harvester = std::thread(serve_harvest_loop, S);
+    S->completer = std::thread(serve_complete_loop, S);
     *out = S;
     return BLAMM_OK;
 }
@@ -296,7 +337,7 @@ int blamm_hip_server_result(blamm_hip_server* S, uint64_t id, blamm_hit* hits, u
                             blamm_request_info* info)
 {
     if (!S) return fail(BLAMM_ERR_ARG, "server_result: null server");
-    std::lock_guard<std::mutex> g(S->mu);
+    std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return fail(BLAMM_ERR_ARG, "server_result: unknown request");
     ServeRequest& q = it->second;
@@ -316,11 +357,13 @@ int blamm_hip_server_result(blamm_hip_server* S, uint64_t id, blamm_hit* hits, u
         return fail(st, "request failed: " + msg);
     }
     if (q.nhits > cap) return fail(BLAMM_ERR_OVERFLOW, "server_result: hit capacity too small");
-    if (q.nhits) {
-        if (!hits) return fail(BLAMM_ERR_ARG, "server_result: null hit buffer");
-        std::memcpy(hits, q.buf->p + q.off, q.nhits * sizeof(blamm_hit));
-    }
+    if (q.nhits && !hits) return fail(BLAMM_ERR_ARG, "server_result: null hit buffer");
+    // the request's slice of the batch's read-back stays alive through buf
+    const std::shared_ptr<HitBuf> buf = q.buf;
+    const uint64_t off = q.off, n = q.nhits;
     S->reqs.erase(it);
+    lk.unlock();
+    if (n) expand_hits(buf->p + off, n, hits);
     return BLAMM_OK;
 }
 
@@ -362,8 +405,10 @@ int blamm_hip_server_destroy(blamm_hip_server* S)
     }
     S->cv_work.notify_all();
     S->cv_flight.notify_all();
+    S->cv_copy.notify_all();
     if (S->launcher.joinable()) S->launcher.join();
     if (S->harvester.joinable()) S->harvester.join();
+    if (S->completer.joinable()) S->completer.join();
     for (ServeSlot& sl : S->slots) blamm_hip_batch_destroy(sl.batch);
     delete S;
     return BLAMM_OK;

@@ -266,7 +266,10 @@ int blamm_hip_batch_query(blamm_hip_batch *batch);
  * when either overflowed); *nhits = hits of all requests. */
 int blamm_hip_batch_wait(blamm_hip_batch *batch, uint64_t *nhits);
 /* Hits of request `req` of the last finished batch, positions local to that
- * request's stream; valid until the next submit. */
+ * request's stream; valid until the next submit.  The device groups a batch's
+ * hits by request and reads them back as 12-byte records (32-bit request-local
+ * position, column, score); the first call after a wait rebuilds them as
+ * blamm_hit. */
 int blamm_hip_batch_hits(blamm_hip_batch *batch, uint32_t req, const blamm_hit **hits, uint64_t *n);
 /* kernel_ms = the whole captured launch sequence of the last batch. */
 int blamm_hip_batch_stats(blamm_hip_batch *batch, blamm_scan_stats *stats);
@@ -278,7 +281,7 @@ int blamm_hip_batch_stats(blamm_hip_batch *batch, blamm_scan_stats *stats);
 typedef struct blamm_server_config {
     uint64_t hbm_budget_bytes;    /* device bytes for batches in flight (0 = 8 GiB),
                                      split evenly over the slots */
-    uint32_t slots;               /* batches in flight, 1..4 (0 = 2) */
+    uint32_t slots;               /* batches in flight, 1..8 (0 = 2) */
     uint32_t max_batch_requests;  /* requests per batch (0 = 256) */
     uint64_t max_batch_bases;     /* bases per batch (0 = 2^24): smaller batches let
                                      one batch's hit read-back overlap the next
@@ -297,14 +300,16 @@ typedef struct blamm_server_stats {
     uint64_t replays;             /* batches run again after growing scratch */
     double device_ms;             /* summed batch launch times */
     double host_submit_ms;        /* launcher thread: staging and launching batches */
-    double host_wait_ms;          /* harvester thread: waits and hit read-back */
+    double host_wait_ms;          /* harvester and completer threads: waits for
+                                     kernels and hit read-back */
 } blamm_server_stats;
 
 typedef struct blamm_hip_server blamm_hip_server;
 
-/* Starts the server's two threads (a launcher that packs queued requests into
- * free batch slots, a harvester that finishes batches oldest first and reads
- * their hits back), which own the batch slots.  While the server runs,
+/* Starts the server's three threads (a launcher that packs queued requests
+ * into free batch slots, a harvester that waits for batches oldest first and
+ * starts their hit read-back, a completer that waits for the read-backs and
+ * finishes the requests), which own the batch slots.  While the server runs,
  * the context's library and tuning must not change. */
 int blamm_hip_server_create(blamm_hip_ctx *ctx, const blamm_server_config *cfg,
                             blamm_hip_server **server);
@@ -316,7 +321,8 @@ int blamm_hip_server_submit(blamm_hip_server *server, const blamm_request *req, 
  * without limit). */
 int blamm_hip_server_wait(blamm_hip_server *server, uint64_t id, int64_t timeout_ms);
 /* Copy the finished request's hits (request-local positions) and timing,
- * then forget the request.  BLAMM_ERR_OVERFLOW (request kept) when cap is
+ * then forget the request (the 12-byte records read back from the device
+ * become blamm_hit here, on the caller's thread).  BLAMM_ERR_OVERFLOW (request kept) when cap is
  * below info->nhits. */
 int blamm_hip_server_result(blamm_hip_server *server, uint64_t id, blamm_hit *hits, uint64_t cap,
                             blamm_request_info *info);

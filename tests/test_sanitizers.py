@@ -9,6 +9,11 @@ functional tests:
 - the occurrence writer (occurrences.cpp: worker-local formatting, pwrite at
   final offsets by one thread per part), whose file must equal the
   single-threaded one;
+- the serving loop (serve.hpp: admission, the launcher / harvester /
+  completer threads, the slot state machine, result hand-out, shutdown with a
+  request still queued) over a CPU stand-in for the batch API
+  (tests/server_harness.cpp), with 8 client threads and 1, 3 and 8 slots;
+  every result must equal the stand-in's hits for that request alone;
 - the `blamm` host code (settings, dict, motif loaders, theoretical
   histograms, .dat writer and reader, thresholds, error paths) under
   ASan + UBSan, with the commands that need no GPU.
@@ -48,11 +53,14 @@ def bins(tmp_path_factory):
     fasta = [os.path.join(REPO, "tests", "fasta_harness.cpp"), os.path.join(HOST, "fasta.cpp")]
     writer = [os.path.join(REPO, "tests", "writer_harness.cpp"), os.path.join(HOST, "occurrences.cpp"),
               os.path.join(HOST, "fasta.cpp"), os.path.join(HOST, "inputs.cpp")]
+    server = [os.path.join(REPO, "tests", "server_harness.cpp")]
     return {
         "fasta_asan": _build(str(d / "fasta_asan"), fasta, ASAN, small),
         "fasta_tsan": _build(str(d / "fasta_tsan"), fasta, TSAN, small),
         "writer_asan": _build(str(d / "writer_asan"), writer, ASAN),
         "writer_tsan": _build(str(d / "writer_tsan"), writer, TSAN),
+        "server_asan": _build(str(d / "server_asan"), server, ASAN, ("-Wno-subobject-linkage",)),
+        "server_tsan": _build(str(d / "server_tsan"), server, TSAN, ("-Wno-subobject-linkage",)),
     }
 
 
@@ -126,6 +134,15 @@ def test_occurrence_writer_under_sanitizers(bins, tmp_path, kind):
             outs.append(open(out, "rb").read())
         assert outs[0] == outs[1]
         assert outs[0].count(b"\n") == expect
+
+
+@pytest.mark.parametrize("kind", ["server_asan", "server_tsan"])
+@pytest.mark.parametrize("slots", [1, 3, 8])
+def test_server_threads_under_sanitizers(bins, kind, slots):
+    """8 client threads x 40 requests through the server's three threads;
+    each result equals the stand-in batch's hits for that request alone."""
+    r = _run([bins[kind], "8", "40", str(slots)])
+    assert r.returncode == 0 and r.stdout.startswith("ok 320 "), (r.stdout, r.stderr[-3000:])
 
 
 @pytest.fixture(scope="module")
