@@ -32,6 +32,7 @@ sys.path.insert(0, REPO)
 PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 MFMA
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 MFMA
 PEAK_HBM_GBS = 8000.0
+PEAK_SCLK_MHZ = 2400.0        # MI355X peak engine clock, at which the MFMA peaks are quoted
 
 
 def log(msg: str) -> None:
@@ -369,7 +370,58 @@ def precision_workload(args):
     return ms, groups
 
 
-def groups_leg(args, groups, precision, steps, warmup, tune_cache=None):
+class ClockSampler:
+    """Samples the GPU's current shader clock (amdgpu sysfs `pp_dpm_sclk`,
+    the level marked `*`) every 50 ms in a thread while running, so the
+    roofline can be stated against the clock the chip actually held.  Only
+    used when exactly one amdgpu card is visible; otherwise reports nothing."""
+
+    def __init__(self):
+        import glob
+        cards = sorted(glob.glob("/sys/class/drm/card[0-9]*/device/pp_dpm_sclk"))
+        self.path = cards[0] if len(cards) == 1 else None
+        self.samples = []
+        self._stop = None
+        self._th = None
+
+    def read(self):
+        try:
+            with open(self.path) as f:
+                for ln in f:
+                    if ln.rstrip().endswith("*"):
+                        return float(ln.split(":")[1].strip().split()[0].lower().replace("mhz", ""))
+        except (OSError, ValueError, IndexError):
+            return None
+        return None
+
+    def __enter__(self):
+        import threading
+        if self.path:
+            self._stop = threading.Event()
+
+            def loop():
+                while not self._stop.wait(0.05):
+                    v = self.read()
+                    if v:
+                        self.samples.append(v)
+            self._th = threading.Thread(target=loop, daemon=True)
+            self._th.start()
+        return self
+
+    def __exit__(self, *exc):
+        if self._th:
+            self._stop.set()
+            self._th.join()
+
+    def summary(self):
+        if not self.samples:
+            return None
+        v = np.asarray(self.samples)
+        return {"source": self.path, "samples": int(v.size), "median_mhz": float(np.median(v)),
+                "min_mhz": float(v.min()), "max_mhz": float(v.max())}
+
+
+def groups_leg(args, groups, precision, steps, warmup, tune_cache=None, clock=None):
     """One context per group with its library resident; `steps` timed passes
     over all groups.  Returns timings and each group's hits."""
     import torch
@@ -394,6 +446,8 @@ def groups_leg(args, groups, precision, steps, warmup, tune_cache=None):
         last = one_pass()
     torch.cuda.synchronize()
     filt, kern = [], []
+    if clock is not None:
+        clock.__enter__()
     t0 = time.perf_counter()
     for _ in range(steps):
         last = one_pass()
@@ -401,6 +455,8 @@ def groups_leg(args, groups, precision, steps, warmup, tune_cache=None):
         kern += [r[2] for r in last]
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    if clock is not None:
+        clock.__exit__(None, None, None)
     out = {"ms_per_step": dt * 1e3, "hits": int(sum(r[0] for r in last)),
            "candidates": int(sum(r[3] for r in last)), "band_rescored": int(sum(r[4] for r in last)),
            "filter_ms_per_launch": float(np.mean(filt)), "kernel_ms_per_group": float(np.mean(kern)),
@@ -831,8 +887,11 @@ def main():
     flops_launch = flops_step / len(groups)
     tune = None if args.no_tune else args.tune_cache
     log("split-bf16 (headline)")
-    bf, bf_hits = groups_leg(args, groups, ba.PREC_BF16X2, args.steps, args.warmup, tune)
+    clock = ClockSampler()
+    bf, bf_hits = groups_leg(args, groups, ba.PREC_BF16X2, args.steps, args.warmup, tune, clock)
     log(f"split-bf16: {bf}")
+    sclk = clock.summary()
+    log(f"shader clock during the timed scans: {sclk}")
     keys_bf = [hit_keys(h) for h in bf_hits]
     value = nbp * ncols / (bf["ms_per_step"] / 1e3) / 1e9
     achieved = flops_launch / (bf["filter_ms_per_launch"] / 1e3) / 1e12
@@ -873,6 +932,11 @@ def main():
                      "traffic_source": "profiles/filter_persistent_config2_traffic.txt"
                                        + ("" if traffic is not None else " (absent or taken on other sources)"),
                      "source_sha": source_hash()},
+        # the published peak assumes the 2.4 GHz peak engine clock; under this
+        # sustained MFMA load the chip holds a lower clock (sampled above)
+        "clock": None if sclk is None else dict(
+            sclk, peak_at_clock_TFLOPs=PEAK_BF16_TFLOPS * sclk["median_mhz"] / PEAK_SCLK_MHZ,
+            frac_at_clock=achieved / (PEAK_BF16_TFLOPS * sclk["median_mhz"] / PEAK_SCLK_MHZ)),
         "hbm": hbm,
         "scan": bf,
     }

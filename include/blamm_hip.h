@@ -105,7 +105,9 @@ int blamm_hip_destroy(blamm_hip_ctx *ctx);
  *               (MotifContainer::generateMatrix, motif.cpp:542-564)
  *   lengths:    motif length of every column (1..BLAMM_MAX_MOTIF_LEN)
  *   thresholds: per-column score threshold; a window is an occurrence iff its
- *               score >= threshold (kernel.cu:29, pwmscan.cpp:117-119)
+ *               score >= threshold (kernel.cu:29, pwmscan.cpp:117-119).  NaN
+ *               is rejected: pass +INFINITY for a column that must report
+ *               nothing (the reference's R >= NaN is never true)
  *   precision:  enum blamm_precision
  * Every weight must be finite.  The hit set is that of a left-to-right FP64
  * sum of the window's weights compared with (double)threshold in every
