@@ -969,31 +969,32 @@ def main():
     for g in groups:
         del g["d_codes"], g["d_runs"]
     sections = [x for x in args.sections.split(",") if x and x != "none"]
-    if "plain" in sections:
+    if "plain" in sections or "hist" in sections:
         log("plain-GPU workload")
         stream, runs, pwms, thr, P, lengths, codes = make_workload(args.seed, args.bases, args.pvalue)
-        p_nbp = int(stream.size)
-        p_flops = float(p_nbp) * 8.0 * float(np.sum(lengths, dtype=np.float64))
-        pb, pb_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_BF16X2, tune_sample=p_nbp)
-        pb["value"] = p_nbp * len(pwms) / (pb["ms_per_step"] / 1e3) / 1e9
-        pach = p_flops / (pb["filter_ms"] / 1e3) / 1e12
-        pb.update({"roofline_achieved_TFLOPs": pach, "roofline_frac": pach / PEAK_BF16_TFLOPS,
-                   "workload": f"plain-GPU config: {len(pwms) // 2} motifs (L 5-30) fwd+rc = {len(pwms)} columns, "
-                               f"{args.bases // 1_000_000} Mbp GC 0.41 CpG 0.2x 0.5% N-runs, p={args.pvalue:g}",
-                   "bases": p_nbp, "columns": len(pwms)})
-        if "host_api" in pb:
-            pb["host_api"]["value"] = p_nbp * len(pwms) / (pb["host_api"]["ms_per_scan"] / 1e3) / 1e9
-            pb["host_api"]["value_pinned"] = p_nbp * len(pwms) / (pb["host_api"]["ms_per_scan_pinned"] / 1e3) / 1e9
-        if not args.no_fp32:
-            pf, pf_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_FP32, tune_sample=16_000_000)
-            pf["value"] = p_nbp * len(pwms) / (pf["ms_per_step"] / 1e3) / 1e9
-            pf["identical_hit_set"] = bool(np.array_equal(hit_keys(pb_hits), hit_keys(pf_hits)))
-            pb["fp32"] = pf
-            pb["speedup_vs_fp32"] = pb["value"] / pf["value"]
-            del pf_hits
-        del pb_hits
-        line["plain_gpu"] = pb
-        log(f"plain: {pb}")
+        if "plain" in sections:
+            p_nbp = int(stream.size)
+            p_flops = float(p_nbp) * 8.0 * float(np.sum(lengths, dtype=np.float64))
+            pb, pb_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_BF16X2, tune_sample=p_nbp)
+            pb["value"] = p_nbp * len(pwms) / (pb["ms_per_step"] / 1e3) / 1e9
+            pach = p_flops / (pb["filter_ms"] / 1e3) / 1e12
+            pb.update({"roofline_achieved_TFLOPs": pach, "roofline_frac": pach / PEAK_BF16_TFLOPS,
+                       "workload": f"plain-GPU config: {len(pwms) // 2} motifs (L 5-30) fwd+rc = {len(pwms)} columns, "
+                                   f"{args.bases // 1_000_000} Mbp GC 0.41 CpG 0.2x 0.5% N-runs, p={args.pvalue:g}",
+                       "bases": p_nbp, "columns": len(pwms)})
+            if "host_api" in pb:
+                pb["host_api"]["value"] = p_nbp * len(pwms) / (pb["host_api"]["ms_per_scan"] / 1e3) / 1e9
+                pb["host_api"]["value_pinned"] = p_nbp * len(pwms) / (pb["host_api"]["ms_per_scan_pinned"] / 1e3) / 1e9
+            if not args.no_fp32:
+                pf, pf_hits = gpu_leg(args, stream, runs, P, lengths, thr, codes, ba.PREC_FP32, tune_sample=16_000_000)
+                pf["value"] = p_nbp * len(pwms) / (pf["ms_per_step"] / 1e3) / 1e9
+                pf["identical_hit_set"] = bool(np.array_equal(hit_keys(pb_hits), hit_keys(pf_hits)))
+                pb["fp32"] = pf
+                pb["speedup_vs_fp32"] = pb["value"] / pf["value"]
+                del pf_hits
+            del pb_hits
+            line["plain_gpu"] = pb
+            log(f"plain: {pb}")
         if "hist" in sections:
             log("hist leg")
             line["hist"] = hist_leg(args, stream, runs)
