@@ -134,9 +134,18 @@ class DeviceScan:
         return {**t.as_dict(), "sample_bases": int(n), "sample_ms": ms}
 
     def step(self, d_codes, n, d_runs, nruns):
+        self.launch(d_codes, n, d_runs, nruns)
+        return self.finish(d_codes, n, d_runs, nruns)
+
+    def launch(self, d_codes, n, d_runs, nruns, stream_ptr=0):
+        """Enqueue the scan (on `stream_ptr`, else the context's stream)."""
+        self.sc.scan_device(d_codes, None, n, d_runs, nruns, self.d_hits, self.cap, self.d_nhits, stream_ptr)
+
+    def finish(self, d_codes, n, d_runs, nruns):
+        """Hit count of the enqueued scan; on an overflow (scratch or hit
+        buffer too small) grow and scan again."""
         ba = self.ba
         while True:
-            self.sc.scan_device(d_codes, None, n, d_runs, nruns, self.d_hits, self.cap, self.d_nhits)
             try:
                 return self.sc.scan_finish()
             except ba.BlammError as e:
@@ -147,6 +156,7 @@ class DeviceScan:
                     self.cap = need + need // 8
                     self.d_hits = self.torch.empty(self.cap * 2, dtype=self.torch.int64, device="cuda:0")
                     self.torch.cuda.synchronize()
+            self.launch(d_codes, n, d_runs, nruns)
 
     def hits(self, nh):
         return self.d_hits[: 2 * nh].cpu().numpy().view(self.ba.HIT_DTYPE)
@@ -434,10 +444,17 @@ def groups_leg(args, groups, precision, steps, warmup, tune_cache=None, clock=No
         for c in ctxs[1:]:
             c.sc.set_tuning(tuning["chunk_tiles"], tuning["blocks_per_wave"], tuning["kpad_positions"])
 
+    # The groups' scans are enqueued back to back on one stream, then
+    # collected: the device runs them in order without a host round trip
+    # between groups (the HIP events around each launch stay per scan).
+    stream = torch.cuda.Stream()
+
     def one_pass():
+        for c, g in zip(ctxs, groups):
+            c.launch(g["d_codes"], g["n"], g["d_runs"], 0, stream.cuda_stream)
         res = []
         for c, g in zip(ctxs, groups):
-            nh = c.step(g["d_codes"], g["n"], g["d_runs"], 0)
+            nh = c.finish(g["d_codes"], g["n"], g["d_runs"], 0)
             st = c.sc.stats()
             res.append((nh, st.filter_ms, st.kernel_ms, int(st.candidates), int(st.band_rescored)))
         return res
