@@ -299,3 +299,104 @@ def test_server_rejects_request_over_budget(scanner):
         assert e.value.code == ba.BLAMM_ERR_ARG
     finally:
         srv.close()
+
+
+def random_request(rng, pwms, base_thr, n):
+    """One request of n bases: random GC, N-run density (dense for some, so
+    run starts fall inside most windows), lowercase share, and a column subset
+    or shifted thresholds."""
+    if n == 0:
+        text, runs = b"", []
+    else:
+        text, runs, _ = stream_case(int(rng.integers(1 << 30)), n,
+                                    lower_every=int(rng.choice([0, 0, 3, 29])),
+                                    nfrac=float(rng.choice([0.0, 0.01, 0.3])), gc=float(rng.uniform(0.3, 0.6)))
+    r = rng.random()
+    if r < 0.25:
+        thr = None
+    elif r < 0.6:
+        thr = base_thr.copy()
+        thr[rng.random(len(thr)) < rng.uniform(0.1, 0.9)] = np.inf
+    else:
+        thr = (base_thr - np.float32(rng.uniform(0.0, 3.0))).astype(np.float32)
+    return text, runs, thr
+
+
+BATCH_SWEEP = int(os.environ.get("BLAMM_BATCH_SWEEP", "6"))
+
+
+@pytest.mark.parametrize("case", range(BATCH_SWEEP))
+def test_batch_random_compositions(scanner, orc, case):
+    """Random libraries (lengths 1-64), random request mixes (sizes from 0 to
+    60 kbp, dense and sparse runs, lowercase, subsets, shifted thresholds) and
+    random tile shapes, three batches in a row on one slot (so buffers grow
+    and the captured sequence is replayed for other shapes): every request
+    equals the FP64 oracle for that request alone."""
+    rng = np.random.default_rng(9000 + case)
+    prec = PRECS[case % 3]
+    pwms, thr, _ = make_library(9100 + case, nmotifs=int(rng.integers(5, 40)), lmin=1,
+                                lmax=int(rng.choice([12, 40, 64])), pvalue=float(rng.choice([1e-2, 1e-3])))
+    P, lengths, _ = ba.build_pattern_matrix(pwms)
+    scanner.set_library(P, lengths, thr, prec)
+    nbmax = 4 if prec == ba.PREC_FP32 else 2
+    tuning = (int(rng.choice([1, 16, 512])), int(rng.choice([b for b in (1, 2, 4) if b <= nbmax])),
+              int(rng.choice([4, 8])))
+    scanner.set_tuning(*tuning)
+    b = ba.Batch(scanner, max_bases=1 << 12, max_requests=2)
+    try:
+        _, err = scanner.bounds()
+        for rnd in range(3):
+            sizes = [int(rng.choice([0, 1, 17, 64, 500, 5000, 60_000])) for _ in range(int(rng.integers(1, 10)))]
+            reqs = [random_request(rng, pwms, thr, n) for n in sizes]
+            say(f"batch sweep case={case} round={rnd} prec={prec} tuning={tuning} sizes={sizes}")
+            b.submit([to_request(*r) for r in reqs])
+            total = b.wait()
+            seen = 0
+            for i, (text, runs, t) in enumerate(reqs):
+                h = b.hits(i)
+                assert_matches(h, oracle_hits(orc, text, runs, pwms, thr if t is None else t), err,
+                               f"case {case} round {rnd} request {i} (n={len(text)})")
+                seen += len(h)
+            assert seen == total
+    finally:
+        b.close()
+        scanner.set_tuning(512, 2, 4)
+
+
+def test_batch_with_more_requests_than_the_lds_tables(scanner, orc):
+    """1,100 requests in one batch: more than finalize and the grouping kernels
+    count in LDS (1,024), so they count with global atomics; every request
+    equals the oracle and the single-request batch's hits."""
+    pwms, thr, _ = make_library(131, nmotifs=12, lmin=4, lmax=20, pvalue=1e-2)
+    P, lengths, _ = ba.build_pattern_matrix(pwms)
+    scanner.set_library(P, lengths, thr, ba.PREC_BF16X2)
+    rng = np.random.default_rng(132)
+    reqs = []
+    for i in range(1100):
+        text, runs, _ = stream_case(5000 + i, int(rng.integers(1, 400)))
+        reqs.append((text, runs, None if i % 3 else (thr - np.float32(0.5)).astype(np.float32)))
+    b = ba.Batch(scanner, max_bases=1 << 20, max_requests=1100)
+    try:
+        _, err = scanner.bounds()
+        say("batch of 1,100 requests")
+        b.submit([to_request(*r) for r in reqs])
+        total = b.wait()
+        assert total > 1000
+        seen = 0
+        sel = list(range(0, 1100, 137))
+        in_batch = {}
+        for i, (text, runs, t) in enumerate(reqs):
+            h = b.hits(i)
+            seen += len(h)
+            if i in sel:
+                in_batch[i] = sorted_hits(h)
+            if i % 50 == 0:
+                assert_matches(h, oracle_hits(orc, text, runs, pwms, thr if t is None else t), err, f"request {i}")
+        assert seen == total
+        # the same requests one per batch give the same hits
+        for i in sel:
+            b.submit([to_request(*reqs[i])])
+            b.wait()
+            assert np.array_equal(sorted_hits(b.hits(0)), in_batch[i]), f"request {i} alone vs in the batch"
+    finally:
+        b.close()

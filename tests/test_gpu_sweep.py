@@ -6,6 +6,7 @@ included), lowercase density, motif lengths 1..64, thresholds from far below
 to above every score and +inf (a skipped column), then checks the scan in both
 precisions and the empirical histograms count for count.
 """
+import os
 import sys
 
 import numpy as np
@@ -15,7 +16,9 @@ import blamm_amd as ba
 
 pytestmark = pytest.mark.gpu
 
-SEEDS = list(range(32))
+# 32 seeds by default; BLAMM_SWEEP_SEEDS=first:count runs another range
+_first, _count = (int(x) for x in os.environ.get("BLAMM_SWEEP_SEEDS", "0:32").split(":"))
+SEEDS = list(range(_first, _first + _count))
 
 
 def say(*a):
