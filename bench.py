@@ -593,7 +593,7 @@ def serve_leg(args):
         return np.asarray(lat), work, span, float(np.mean(batch))
 
     run([0.0] * len(pool), pool)   # warm-up: captures the slot graphs, sizes scratch and pinned buffers
-    n_closed = 2 * len(pool)
+    n_closed = 4 * len(pool)
     s0 = srv.stats()
     lat, work, span, mb = run([0.0] * n_closed, pool)
     s1 = srv.stats()

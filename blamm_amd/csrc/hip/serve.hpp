@@ -278,6 +278,7 @@ int blamm_hip_server_create(blamm_hip_ctx* c, const blamm_server_config* cfg, bl
             delete S;
             return rc;
         }
+        sl.batch->leave_room = k.slots > 1;
     }
     S->launcher = std::thread(serve_launch_loop, S);
     S->harvester = std::thread(serve_harvest_loop, S);

@@ -83,6 +83,7 @@ struct blamm_hip_batch {
     std::vector<uint64_t> out_off;
     uint64_t replays = 0;
     bool pending = false;
+    bool leave_room = false;
 };
 
 namespace {
