@@ -182,7 +182,7 @@ struct blamm_hip_ctx {
     std::vector<double> band_h, seq_h;         // per slot
     double max_filter_band = 0.0, max_score_error = 0.0;
     DevBuf<uint8_t> frag_a, frag_b;
-    DevBuf<float> slot_thr, slot_w;
+    DevBuf<float> slot_thr, slot_w, slot_w2;   // w2: FP32 sums of position pairs
     DevBuf<double> slot_band, slot_band_any, slot_seq;   // band_any: valid in a block of any length
     DevBuf<double> slot_band_hi, slot_band_hi_any;      // hi-only stage of the bf16 filter
     DevBuf<int> slot_len;
@@ -475,6 +475,17 @@ int relayout(blamm_hip_ctx* c)
     HIP_TRY(c->slot_band_hi_any.reserve(nslots));
     HIP_TRY(c->slot_seq.reserve(nslots));
     HIP_TRY(c->slot_w.reserve(s_w.size()));
+    // finalize's pair table: entry b1 + 4 b2 of pair q is w[2q][b1] + w[2q+1][b2]
+    // rounded to FP32 (rows past a motif's length are zero)
+    std::vector<float> s_w2((size_t)nslots * (kMaxLen / 2) * 16, 0.f);
+    for (size_t sl = 0; sl < (size_t)nslots; ++sl)
+        for (int q = 0; q < kMaxLen / 2; ++q)
+            for (int b1 = 0; b1 < 4; ++b1)
+                for (int b2 = 0; b2 < 4; ++b2)
+                    s_w2[(sl * (kMaxLen / 2) + q) * 16 + b1 + 4 * b2] =
+                        s_w[(sl * kMaxLen + 2 * q) * 4 + b1] + s_w[(sl * kMaxLen + 2 * q + 1) * 4 + b2];
+    HIP_TRY(c->slot_w2.reserve(s_w2.size()));
+    HIP_TRY(hipMemcpy(c->slot_w2.p, s_w2.data(), s_w2.size() * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(c->slot_len.reserve(nslots));
     HIP_TRY(c->slot_col.reserve(nslots));
     HIP_TRY(c->groups_d.reserve(gpk.size()));
@@ -538,6 +549,7 @@ FinalizeArgs finalize_args(const blamm_hip_ctx* c, const uint4* cand, uint64_t c
     f.run_bits = nullptr;
     f.slot_len = c->slot_len.p;
     f.slot_w = c->slot_w.p;
+    f.slot_w2 = c->slot_w2.p;
     f.slot_thr = c->slot_thr.p;
     f.slot_seq = c->slot_seq.p;
     f.slot_col = c->slot_col.p;
@@ -618,7 +630,7 @@ int blamm_hip_destroy(blamm_hip_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->frag_a.release(); c->frag_b.release(); c->slot_thr.release(); c->slot_band.release();
     c->slot_band_any.release(); c->slot_band_hi.release(); c->slot_band_hi_any.release();
-    c->slot_seq.release(); c->slot_w.release();
+    c->slot_seq.release(); c->slot_w.release(); c->slot_w2.release();
     c->slot_len.release(); c->slot_col.release(); c->groups_d.release(); c->cand.release();
     c->counters.release(); c->run_bits.release();
     c->packed.release(); c->lcmask.release(); c->runs.release(); c->nhits_dev.release();

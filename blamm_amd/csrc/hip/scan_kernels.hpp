@@ -751,6 +751,7 @@ struct FinalizeArgs {
     const uint64_t* run_bits;             // bit i set: a run starts at base i (null: search run_starts)
     const int* slot_len;
     const float* slot_w;                  // [slot][kMaxLen][4] FP32 weights
+    const float* slot_w2;                 // [slot][kMaxLen / 2][16] FP32 sums of position pairs
     const float* slot_thr;                // the threshold per slot (list form: per batch slot)
     const double* slot_seq;               // sequential-FP32 band per slot
     const uint32_t* slot_col;             // caller's column index
@@ -883,6 +884,7 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
         if (!have) rows = 0;
         const int L = have ? a.slot_len[lib] : 0;
         const float* w = a.slot_w + (size_t)lib * kMaxLen * 4;
+        const float* w2 = a.slot_w2 + (size_t)lib * (kMaxLen / 2) * 16;
         uint32_t nb1 = 0;
         if (have && !a.run_bits) {
             // first run start after q0
@@ -913,11 +915,30 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(FinalizeArgs a)
                 if (L > 32) bits2 = load_bits64(a.packed, 2ull * q + 64);
                 if (a.lcmask) mbits = load_bits64(a.lcmask, q);
             }
-            // left-to-right FP32 sum of the window: each lane loads only the
-            // weight of its base, four positions at a time so the loads
-            // overlap (rows up to kMaxLen exist for every slot)
+            // FP32 sum of the window over its positions in order.  Without a
+            // lowercase base in the window it adds one table entry per pair of
+            // positions (the FP32 sum of the pair's two weights, indexed by the
+            // pair's 4 bits of packed bases; a zero weight row completes an odd
+            // length), so each lane makes half the dependent weight loads; with
+            // one it adds the weights of the uppercase positions one by one.
+            // Either way |sum - exact| <= gamma_(L-1) sum_p max_b |w|, which
+            // the sequential band (slot_seq) covers.  Four loads at a time so
+            // they overlap (rows up to kMaxLen exist for every slot).
             float s = 0.f;
             const int Lw = act ? L : 0;
+            const uint64_t lenmask = L >= 64 ? ~0ull : (1ull << L) - 1ull;
+            if (!(mbits & lenmask)) {
+                const int np = (Lw + 1) >> 1;
+                for (int q0 = 0; q0 < np; q0 += 4) {
+                    const uint64_t src = q0 < 16 ? bits >> (4 * q0) : bits2 >> (4 * (q0 - 16));
+                    float x[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) x[u] = w2[16 * (q0 + u) + (uint32_t)((src >> (4 * u)) & 15u)];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (q0 + u < np) s += x[u];
+                }
+            } else
             for (int p0 = 0; p0 < Lw; p0 += 4) {
                 const uint64_t src = p0 < 32 ? bits >> (2 * p0) : bits2 >> (2 * (p0 - 32));
                 float x[4];
