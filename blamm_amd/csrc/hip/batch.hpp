@@ -91,12 +91,34 @@ int acquire_hits(const std::shared_ptr<HitPool>& pool, size_t want, std::shared_
 // for, and the completer waits for the copy's completion signal with acquire
 // semantics.  When the agents cannot be resolved the batch keeps
 // hipMemcpyAsync.
+//
+// Engine choice: MI355X has 16 SDMA engines; the agent reports the first
+// HSA_AMD_AGENT_INFO_NUM_SDMA_ENG (2) as the engines for host traffic and the
+// rest as xGMI engines.  Measured alone, device-to-host: engines 0-3 ~56 GB/s,
+// 4-7 12.7 GB/s, 8-15 7-10 GB/s.  Measured under a serving run (3 slots,
+// p = 1e-3): engines 0, 2 and 3 56.8 GB/s, engine 1 (the runtime's preferred
+// device-to-host engine, where its own queues sit) 30 GB/s.  Left to the
+// runtime, the read-backs moved between such engines and ran at ~30 GB/s for
+// long stretches.  They are therefore issued on the first host engine;
+// without an engine count the runtime picks.
 struct DmaReadback {
     bool ready = false;
     hsa_agent_t gpu{}, cpu{};
     hsa_signal_t sig{};
     uint64_t ticks_per_s = 0;
+    uint32_t engine = 0;   // hsa_amd_sdma_engine_id_t bit, 0 = the runtime's choice
 };
+
+// Start copying `bytes` from device memory to pinned host memory; completion
+// brings d.sig from 1 to 0.
+hsa_status_t dma_start(DmaReadback& d, void* dst, const void* src, size_t bytes)
+{
+    hsa_signal_store_screlease(d.sig, 1);
+    if (d.engine)
+        return hsa_amd_memory_async_copy_on_engine(dst, d.cpu, src, d.gpu, bytes, 0, nullptr, d.sig,
+                                                   (hsa_amd_sdma_engine_id_t)d.engine, false);
+    return hsa_amd_memory_async_copy(dst, d.cpu, src, d.gpu, bytes, 0, nullptr, d.sig);
+}
 
 hsa_status_t find_cpu_agent(hsa_agent_t a, void* out)
 {
@@ -127,6 +149,14 @@ bool dma_setup(DmaReadback& d, const void* device_ptr)
         d.ticks_per_s == 0)
         return false;
     if (hsa_signal_create(0, 0, nullptr, &d.sig) != HSA_STATUS_SUCCESS) return false;
+    d.engine = 0;
+    uint32_t nhost = 0, avail = 0;
+    if (hsa_agent_get_info(d.gpu, (hsa_agent_info_t)HSA_AMD_AGENT_INFO_NUM_SDMA_ENG, &nhost) == HSA_STATUS_SUCCESS &&
+        nhost > 0 && nhost <= 16 &&
+        hsa_amd_memory_copy_engine_status(d.cpu, d.gpu, &avail) == HSA_STATUS_SUCCESS) {
+        const uint32_t host = ((1u << nhost) - 1u) & avail;
+        d.engine = host & (~host + 1u);   // first available host engine (0 if none)
+    }
     d.ready = true;
     return true;
 }
@@ -348,8 +378,22 @@ int batch_enqueue_kernels(blamm_hip_batch* b, bool mask, uint32_t grid)
     return BLAMM_OK;
 }
 
+// The batch's counters and per-request hit offsets, stored by one workgroup
+// straight into pinned host memory once the batch's kernels are done (the
+// stream orders it after them; the completion event after it releases the
+// stores to the host).  Two small hipMemcpyAsync read-backs would instead sit
+// in one of the runtime's DMA queues until the kernels end; this way the hit
+// read-back is the batch's only work on a DMA engine.
+__global__ __launch_bounds__(256) void export_counts_kernel(const unsigned long long* counters,
+                                                            const unsigned long long* req_off, uint32_t nreq,
+                                                            unsigned long long* cnt_out, unsigned long long* off_out)
+{
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kCntSlots; i += blockDim.x) cnt_out[i] = counters[i];
+    for (uint32_t i = threadIdx.x; i <= nreq; i += blockDim.x) off_out[i] = req_off[i];
+}
+
 // Launch the captured sequence (capturing it first when buffers or the
-// library moved since the last capture), then copy the counters back.
+// library moved since the last capture), then export the counters.
 int batch_launch(blamm_hip_batch* b)
 {
     blamm_hip_ctx* c = b->ctx;
@@ -381,10 +425,11 @@ int batch_launch(blamm_hip_batch* b)
     HIP_TRY(hipEventRecord(b->ev_start, b->stream));
     HIP_TRY(hipGraphLaunch(b->exec[v], b->stream));
     HIP_TRY(hipEventRecord(b->ev_end, b->stream));
-    HIP_TRY(hipMemcpyAsync(b->cnt_h.p, b->counters.p, kCntSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                           b->stream));
-    HIP_TRY(hipMemcpyAsync(b->off_h.p, b->req_off.p, ((size_t)b->nreq + 1) * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, b->stream));
+    if (b->off_h.n < (size_t)b->nreq + 1 || b->req_off.n < (size_t)b->nreq + 1 || b->cnt_h.n < (size_t)kCntSlots)
+        return fail(BLAMM_ERR_STATE, "batch launch: counter buffers smaller than the batch");
+    hipLaunchKernelGGL(export_counts_kernel, dim3(1), dim3(256), 0, b->stream, b->counters.p, b->req_off.p, b->nreq,
+                       b->cnt_h.p, b->off_h.p);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(b->ev_done, b->stream));
     return BLAMM_OK;
 }
@@ -436,9 +481,7 @@ int batch_collect(blamm_hip_batch* b, uint64_t* nhits)
     b->copy_dma = false;
     int rc = acquire_hits(b->pool, std::max<uint64_t>(nh, 1), &b->copying);
     if (rc == BLAMM_OK && nh > 0 && b->dma.ready) {
-        hsa_signal_store_screlease(b->dma.sig, 1);
-        if (hsa_amd_memory_async_copy(b->copying->p, b->dma.cpu, b->grouped.p, b->dma.gpu,
-                                      nh * sizeof(GroupedHit), 0, nullptr, b->dma.sig) == HSA_STATUS_SUCCESS) {
+        if (dma_start(b->dma, b->copying->p, b->grouped.p, nh * sizeof(GroupedHit)) == HSA_STATUS_SUCCESS) {
             b->copy_dma = true;
         } else {
             hsa_signal_store_screlease(b->dma.sig, 0);
