@@ -26,8 +26,8 @@ struct ColumnInfo {
     char strand;
 };
 
-// The occurrence file.  Each batch of formatted lines arrives as one string
-// per worker; the workers' strings are written at their final offsets by one
+// The occurrence file.  Each batch of formatted lines arrives as one buffer
+// per worker; the workers' buffers are written at their final offsets by one
 // thread each (pwrite), so the copy into the page cache runs in parallel.  A
 // destination without offsets (a pipe) is written in order.
 class OutFile {
@@ -45,21 +45,26 @@ public:
     OutFile(const OutFile&) = delete;
     OutFile& operator=(const OutFile&) = delete;
 
-    void write_parts(const std::vector<std::string>& parts)
+    struct Part {
+        const char* p = nullptr;
+        size_t n = 0;
+    };
+
+    void write_parts(const std::vector<Part>& parts)
     {
         if (!seekable_) {
-            for (const std::string& p : parts) put(p.data(), p.size(), -1);
+            for (const Part& p : parts) put(p.p, p.n, -1);
             return;
         }
         std::vector<uint64_t> at(parts.size());
         for (size_t i = 0; i < parts.size(); ++i) {
             at[i] = off_;
-            off_ += parts[i].size();
+            off_ += parts[i].n;
         }
         std::vector<std::string> err(parts.size());
         auto one = [&](size_t i) {
             try {
-                put(parts[i].data(), parts[i].size(), (int64_t)at[i]);
+                put(parts[i].p, parts[i].n, (int64_t)at[i]);
             } catch (const std::exception& e) {
                 err[i] = e.what();
             }
@@ -71,6 +76,16 @@ public:
         for (const std::string& e : err)
             if (!e.empty()) throw std::runtime_error(e);
     }
+
+    // For writers that place parts themselves: whether the file takes
+    // offsets, the offset the next part starts at, a write of [p, p + n) at
+    // offset `at` (thread-safe) or at the current position (a pipe), and
+    // moving the next-part offset past what was written at offsets.
+    bool seekable() const { return seekable_; }
+    uint64_t offset() const { return off_; }
+    void write_at(const char* p, size_t n, uint64_t at) { put(p, n, (int64_t)at); }
+    void write_next(const char* p, size_t n) { put(p, n, -1); }
+    void advance(uint64_t n) { off_ += n; }
 
     void close()
     {
