@@ -9,6 +9,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <charconv>
 #include <chrono>
@@ -457,6 +458,35 @@ int scan_main(int argc, char** argv)
         std::vector<float> P(ldp * ncols, 0.0f), thr(ncols);
         for (size_t c = 0; c < ncols; ++c) pfm_to_pwm(motifs[c], g.counts, st.pseudocount, &P[c * ldp]);
         std::map<std::string, float> pcut;   // one histogram per base name (fwd and rc share it)
+        if (!absSpec && !relSpec) {
+            // the .dat files load in parallel; the first one (in column order)
+            // that fails is the one reported
+            std::vector<std::string> names;
+            std::set<std::string> seen;
+            for (size_t c = 0; c < ncols; ++c)
+                if (seen.insert(motifs[c].baseName()).second) names.push_back(motifs[c].baseName());
+            std::vector<float> cut(names.size());
+            std::vector<std::string> errs(names.size());
+            std::atomic<size_t> next{0};
+            auto load = [&]() {
+                for (size_t i; (i = next++) < names.size();) {
+                    try {
+                        cut[i] = Histogram::load(histdir + "hist_" + g.name + "_" + names[i] + ".dat").cutoff(pvalue);
+                    } catch (const std::exception& e) {
+                        errs[i] = e.what();
+                    }
+                }
+            };
+            std::vector<std::thread> pool;
+            const size_t nt = std::min<size_t>(numThreads, (names.size() + 15) / 16);
+            for (size_t t = 1; t < nt; ++t) pool.emplace_back(load);
+            load();
+            for (auto& th : pool) th.join();
+            for (size_t i = 0; i < names.size(); ++i) {
+                if (!errs[i].empty()) throw std::runtime_error(errs[i]);
+                pcut.emplace(names[i], cut[i]);
+            }
+        }
         for (size_t c = 0; c < ncols; ++c) {
             const float* w = &P[c * ldp];
             if (absSpec) {
@@ -465,18 +495,14 @@ int scan_main(int argc, char** argv)
                 const float mx = max_score(w, motifs[c].size()), mn = min_score(w, motifs[c].size());
                 thr[c] = relTh * (mx - mn) + mn;
             } else {
-                const std::string bn = motifs[c].baseName();
-                auto it = pcut.find(bn);
-                if (it == pcut.end())
-                    it = pcut.emplace(bn, Histogram::load(histdir + "hist_" + g.name + "_" + bn + ".dat").cutoff(pvalue)).first;
-                thr[c] = it->second;
+                thr[c] = pcut.at(motifs[c].baseName());
             }
         }
         for (size_t c = 0; c < ncols; ++c) {
             if (motifs[c].size() > 15) continue;
             const float* w = &P[c * ldp];
             cutoffs << g.name << "\t" << motifs[c].name << "\t" << min_score(w, motifs[c].size()) << "\t" << thr[c]
-                    << "\t" << max_score(w, motifs[c].size()) << std::endl;
+                    << "\t" << max_score(w, motifs[c].size()) << '\n';
         }
         clock.tick("thresholds");
         const PackedStream s = pack_fasta(g.files, g.totSeqLen);
