@@ -719,6 +719,7 @@ int blamm_hip_reserve(blamm_hip_ctx* c, uint64_t n)
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(c->cand.reserve(n));
+    HIP_TRY(c->hits.reserve(n));
     return BLAMM_OK;
 }
 

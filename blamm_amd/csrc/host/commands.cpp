@@ -525,6 +525,14 @@ int scan_main(int argc, char** argv)
         // ones proportionally shorter, which bounds the pinned hit buffer.
         uint64_t seg = segment_bases();
         const uint64_t halo = lmax - 1;
+        if (!absSpec && !relSpec && s.n > 0) {
+            // -pt thresholds pass about p of a column's windows: room for three
+            // times that many records and hits in a segment (at most what a
+            // segment may leave), so the first scan usually runs once
+            const double est = 3.0 * (double)pvalue * (double)std::min<uint64_t>(seg, s.n) * (double)ncols;
+            const double cap = 1.25 * (double)kSegmentHits;
+            hip_check(blamm_hip_reserve(hc.ctx, (uint64_t)std::min(est, cap) + (1u << 16)), "scratch reservation");
+        }
         if (!tuneCache.empty() && s.n > 0) {
             // tile shapes from a sample of this group (the first 16 Mbp)
             const uint64_t m = std::min<uint64_t>(s.n, 1ull << 24);

@@ -125,8 +125,10 @@ int blamm_hip_library_bounds(blamm_hip_ctx *ctx, double *max_filter_band,
 uint64_t blamm_hip_packed_words(uint64_t nbases);
 uint64_t blamm_hip_mask_words(uint64_t nbases);
 
-/* Reserve room for n filter candidates (device scratch).  Scans grow it on
- * demand; reserving up front keeps device allocation out of later calls.
+/* Reserve room for n filter candidates (device scratch) and, for the
+ * host-buffer scans, n hits in the context's device hit buffer (16 bytes
+ * each, both).  Scans grow them on demand; reserving up front keeps device
+ * allocation out of later calls and lets a first scan that fits run once.
  * BLAMM_ERR_STATE while a scan_device is in flight. */
 int blamm_hip_reserve(blamm_hip_ctx *ctx, uint64_t n);
 
