@@ -161,6 +161,21 @@ bool dma_setup(DmaReadback& d, const void* device_ptr)
     return true;
 }
 
+// Gathered fragments per column block of a batch (uint4 elements of frag_a,
+// and of frag_b in the split-BF16 forms).
+size_t batch_frag_elems(bool bf)
+{
+    return bf ? (size_t)kMaxChunks * 64 : (size_t)kMaxLen * 16;
+}
+
+// Device bytes of one column block of a batch: its fragments and the
+// per-slot threshold, slot map and band tables.
+uint64_t batch_block_bytes(const blamm_hip_ctx* c)
+{
+    const bool bf = prec_bf(c->precision);
+    return (bf ? 2 : 1) * batch_frag_elems(bf) * sizeof(uint4) + (uint64_t)c->cols_per_blk * (4 + 4 + 8 + 8);
+}
+
 // Host-written tables of a batch live in one device arena laid out like their
 // pinned staging image, so that one copy uploads them all.  Each table has a
 // capacity; a batch that needs more grows the table geometrically (the arena
@@ -757,7 +772,7 @@ int blamm_hip_batch_submit(blamm_hip_batch* b, const blamm_request* reqs, uint32
     BLAMM_TRY(meta_fit(b, need));
     BLAMM_TRY(batch_grow(b, b->band, nbs));
     BLAMM_TRY(batch_grow(b, b->band_hi, nbs));
-    const size_t fr = (size_t)std::max<uint32_t>(nblk, 1) * (bf ? (size_t)kMaxChunks * 64 : (size_t)kMaxLen * 16);
+    const size_t fr = (size_t)std::max<uint32_t>(nblk, 1) * batch_frag_elems(bf);
     BLAMM_TRY(batch_grow(b, b->frag_a, fr));
     if (bf) BLAMM_TRY(batch_grow(b, b->frag_b, fr));
     else BLAMM_TRY(batch_grow(b, b->frag_b, 1));

@@ -74,8 +74,10 @@ struct blamm_hip_server {
 
 namespace {
 
-// Device bytes a request occupies inside a batch (inputs, its threshold
-// tables, work items, and its expected candidates and hits).
+// Device bytes a request occupies inside a batch: its stream, mask, run
+// starts and run-start bits; its column blocks (fragments and per-slot
+// tables); its work items; and per expected hit a filter record (16 B, about
+// one per hit), a finalize output (16 B) and a grouped read-back record.
 uint64_t request_bytes(const blamm_hip_ctx* c, const blamm_request& q)
 {
     const uint64_t n = q.nbases + kReqAlign;
@@ -84,8 +86,11 @@ uint64_t request_bytes(const blamm_hip_ctx* c, const blamm_request& q)
         if (!q.thresholds || q.thresholds[j] != INFINITY) ++active;
     const uint64_t hits = q.expected_hits ? q.expected_hits : (uint64_t)(1e-4 * (double)q.nbases * (double)active) + 1;
     const uint64_t tiles = n / kBf16Tile + 1;
-    return n / 4 + (q.lcmask ? n / 8 : 0) + 8 * (q.nruns + 1) + 16 * (uint64_t)c->nslots +
-           8 * (tiles / std::max<uint32_t>(1, c->tuning.chunk_tiles) + 1) * c->groups.size() + 24 * hits;
+    const uint64_t cpb = (uint64_t)std::max(1, c->cols_per_blk);
+    const uint64_t blocks = (active + cpb - 1) / cpb;
+    return n / 4 + (q.lcmask ? n / 8 : 0) + 8 * (q.nruns + 1) + n / 8 + blocks * batch_block_bytes(c) +
+           8 * (tiles / std::max<uint32_t>(1, c->tuning.chunk_tiles) + 1) * c->groups.size() +
+           (32 + sizeof(GroupedHit)) * hits;
 }
 
 void serve_finish(blamm_hip_server* S, ServeSlot& sl, int rc, std::unique_lock<std::mutex>& lk)

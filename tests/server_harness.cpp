@@ -71,7 +71,7 @@ std::vector<GroupedHit> fake_hits(const blamm_request& q, int ncols)
 struct blamm_hip_ctx {
     int device = 0;
     bool have_lib = true;
-    int ncols = 6, nslots = 6;
+    int ncols = 6, nslots = 6, cols_per_blk = 32;
     blamm_tuning tuning{512, 2, 4};
     std::vector<int> groups = std::vector<int>(3);
 };
@@ -87,6 +87,8 @@ struct blamm_hip_batch {
 };
 
 namespace {
+uint64_t batch_block_bytes(const blamm_hip_ctx*) { return 32768; }
+
 std::shared_ptr<HitBuf> new_hitbuf(size_t n)
 {
     HitBuf* h = new HitBuf;
