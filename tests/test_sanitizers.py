@@ -120,10 +120,28 @@ def _random_writer_case(tmp_path, seed):
     return fa, hb, payload, int((hits["pos"] < payload).sum())
 
 
+def _expected_lines(fa, hb, payload):
+    """The occurrence lines of the writer harness's hits, built from the
+    oracle's FASTA filter (run starts -> record and position) and the
+    harness's column names, lengths and strands; scores as the reference's
+    ostream prints a float (%g of the value as a double)."""
+    _, rs, rq, rp, _ = __import__("oracle").default().filter_fasta([fa])
+    hits = np.fromfile(hb, ba.HIT_DTYPE)
+    out = []
+    for pos, col, score in zip(hits["pos"].tolist(), hits["col"].tolist(), hits["score"].tolist()):
+        if pos >= payload:
+            continue
+        r = int(np.searchsorted(rs, pos, side="right")) - 1
+        p = int(rp[r]) + pos - int(rs[r])
+        ln = 5 + col % 7
+        out.append(f"rec{int(rq[r])}\tblamm\tM{col}\t{p}\t{p + ln}\t{'%g' % score}\t{'-' if col % 2 else '+'}\t.\t.\n")
+    return "".join(out).encode()
+
+
 @pytest.mark.parametrize("kind", ["writer_asan", "writer_tsan"])
 def test_occurrence_writer_under_sanitizers(bins, tmp_path, kind):
-    """8 writer threads produce the same file as 1; the line count is the
-    number of hits below the payload."""
+    """8 writer threads produce the same file as 1, and that file is the
+    lines built independently from the oracle's FASTA filter."""
     for seed in range(6):
         fa, hb, payload, expect = _random_writer_case(tmp_path, seed)
         outs = []
@@ -134,6 +152,7 @@ def test_occurrence_writer_under_sanitizers(bins, tmp_path, kind):
             outs.append(open(out, "rb").read())
         assert outs[0] == outs[1]
         assert outs[0].count(b"\n") == expect
+        assert outs[0] == _expected_lines(fa, hb, payload), seed
 
 
 @pytest.mark.parametrize("kind", ["server_asan", "server_tsan"])
