@@ -120,9 +120,30 @@ uint64_t PackedStream::nextBreak(uint64_t q) const
 
 namespace {
 
+// Bytes of a segment: a grow-only buffer that is not zero-filled (the reads
+// write every byte before it is used), reused from segment to segment.
+struct SegmentBytes {
+    std::unique_ptr<char[]> p;
+    size_t n = 0, cap = 0;
+    const char* data() const { return p.get(); }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+    char operator[](size_t i) const { return p[i]; }
+    // room for `want` bytes, keeping the first n
+    void reserve(size_t want)
+    {
+        if (want <= cap) return;
+        const size_t c = std::max(want, cap + cap / 2);
+        std::unique_ptr<char[]> q(new char[c]);
+        if (n) std::memcpy(q.get(), p.get(), n);
+        p = std::move(q);
+        cap = c;
+    }
+};
+
 // A segment of a file (whole lines) cut into line-aligned chunks.
 struct FileData {
-    std::string bytes;
+    SegmentBytes bytes;
     std::vector<std::pair<size_t, size_t>> chunks;   // [begin, end) at line starts
 };
 
@@ -151,26 +172,34 @@ public:
     bool next(FileData& d)
     {
         d.chunks.clear();
-        d.bytes.swap(carry_);
+        // the previous segment's unfinished last line comes first
+        d.bytes.n = 0;
+        d.bytes.reserve(carry_.size());
+        if (!carry_.empty()) std::memcpy(d.bytes.p.get(), carry_.data(), carry_.size());
+        d.bytes.n = carry_.size();
         carry_.clear();
         if (d.bytes.empty() && off_ >= size_) return false;
         size_t cut = std::string::npos;
         while (off_ < size_) {
             const size_t want = std::min<size_t>(BLAMM_FASTA_SEGMENT, size_ - off_);
-            const size_t base = d.bytes.size();
-            d.bytes.resize(base + want);
-            read_at(&d.bytes[base], want, off_);
+            const size_t base = d.bytes.n;
+            d.bytes.reserve(base + want);
+            read_at(d.bytes.p.get() + base, want, off_);
+            d.bytes.n = base + want;
             off_ += want;
-            const size_t nl = d.bytes.rfind('\n');
-            if (nl != std::string::npos && nl >= base) {
-                cut = nl + 1;
+            // last line end among the new bytes
+            const char* q = d.bytes.p.get();
+            size_t nl = base + want;
+            while (nl > base && q[nl - 1] != '\n') --nl;
+            if (nl > base) {
+                cut = nl;
                 break;
             }
             // no line end in the new bytes: a line longer than a segment
         }
         if (off_ < size_ && cut != std::string::npos) {
-            carry_.assign(d.bytes, cut, std::string::npos);
-            d.bytes.resize(cut);
+            carry_.assign(d.bytes.data() + cut, d.bytes.n - cut);
+            d.bytes.n = cut;
         }
         const size_t size = d.bytes.size();
         const size_t chunk = std::max<size_t>(size / (4 * threads_) + 1, BLAMM_FASTA_MIN_CHUNK);
