@@ -325,12 +325,29 @@ int hist_main(int argc, char** argv)
             empirical_histograms(hc, device, g, motifs, st.pseudocount, numBins, maxLength, lmax, histdir);
             continue;
         }
-        for (const Motif& m : motifs) {
-            std::vector<float> w(4 * m.size());
-            pfm_to_pwm(m, g.counts, st.pseudocount, w.data());
-            const Histogram h = theoretical_histogram(w.data(), m.size(), bg, numBins, maxLength);
-            h.write(histdir, "hist_" + g.name + "_" + m.name, m.name + " (" + g.name + ")");
-        }
+        // one motif per task on numThreads workers (each writes its own
+        // files); the first failure in motif order is the one reported
+        std::vector<std::string> errs(motifs.size());
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i; (i = next++) < motifs.size();) {
+                const Motif& m = motifs[i];
+                try {
+                    std::vector<float> w(4 * m.size());
+                    pfm_to_pwm(m, g.counts, st.pseudocount, w.data());
+                    const Histogram h = theoretical_histogram(w.data(), m.size(), bg, numBins, maxLength);
+                    h.write(histdir, "hist_" + g.name + "_" + m.name, m.name + " (" + g.name + ")");
+                } catch (const std::exception& e) {
+                    errs[i] = e.what();
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < std::min(numThreads, motifs.size()); ++t) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
+        for (const std::string& e : errs)
+            if (!e.empty()) throw std::runtime_error(e);
     }
     return EXIT_SUCCESS;
 }
