@@ -61,11 +61,17 @@ std::vector<Group> build_dict(const std::string& manifest)
         std::cout.flush();
         groups[g].files.assign(files[g].begin(), files[g].end());
         const PackedStream s = pack_fasta(groups[g].files, UINT64_MAX);
+        // per packed word (32 bases, A=00 C=01 G=10 T=11): the low and high
+        // code bits at each base's even bit position, counted by popcount
         std::array<uint64_t, 4> cnt{{0, 0, 0, 0}};
-        for (uint64_t w = 0; w < s.codes.size(); ++w) {
-            const uint64_t word = s.codes[w];
+        for (uint64_t w = 0; w < s.codes.size() && 32 * w < s.n; ++w) {
             const uint64_t nb = std::min<uint64_t>(32, s.n - 32 * w);
-            for (uint64_t i = 0; i < nb; ++i) ++cnt[(word >> (2 * i)) & 3];
+            const uint64_t m = nb == 32 ? 0x5555555555555555ull : 0x5555555555555555ull & ((1ull << (2 * nb)) - 1);
+            const uint64_t lo = s.codes[w] & m, hi = (s.codes[w] >> 1) & m;
+            cnt[0] += (uint64_t)__builtin_popcountll(m & ~(lo | hi));
+            cnt[1] += (uint64_t)__builtin_popcountll(lo & ~hi);
+            cnt[2] += (uint64_t)__builtin_popcountll(hi & ~lo);
+            cnt[3] += (uint64_t)__builtin_popcountll(lo & hi);
         }
         groups[g].counts = cnt;
         groups[g].totSeqLen = cnt[0] + cnt[1] + cnt[2] + cnt[3];
