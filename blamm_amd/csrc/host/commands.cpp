@@ -160,6 +160,40 @@ struct HipContext {
     }
 };
 
+// Opens the device on a thread of its own (the HIP runtime's start-up and the
+// library's code object load take ~50 ms) while the caller reads its inputs;
+// take() waits for it.  Whatever it opened and nobody took is closed.
+struct DeviceOpener {
+    std::thread th;
+    blamm_hip_ctx* ctx = nullptr;
+    int ndev = 0, rc = BLAMM_OK;
+    std::string err;
+    void start(int device)
+    {
+        th = std::thread([this, device] {
+            blamm_hip_device_count(&ndev);
+            if (ndev == 0) return;
+            rc = blamm_hip_create(device, &ctx);
+            if (rc != BLAMM_OK) err = blamm_hip_last_error();   // the message is per thread
+        });
+    }
+    // the opened context; throws the errors the opening met
+    blamm_hip_ctx* take(const char* what)
+    {
+        if (th.joinable()) th.join();
+        if (ndev == 0) throw std::runtime_error("HIP error: no devices found. Aborting...");
+        if (rc != BLAMM_OK) throw std::runtime_error(std::string(what) + ": " + err);
+        blamm_hip_ctx* c = ctx;
+        ctx = nullptr;
+        return c;
+    }
+    ~DeviceOpener()
+    {
+        if (th.joinable()) th.join();
+        if (ctx) blamm_hip_destroy(ctx);
+    }
+};
+
 // Bases per scan / histogram segment: 2^28, or BLAMM_SEGMENT_BASES rounded
 // down to a multiple of 64 (at least 64).  Segments are cut at multiples of
 // 64 bases so that their packed words and lowercase words start on word
@@ -420,6 +454,12 @@ int scan_main(int argc, char** argv)
     if (absSpec && pSpec) throw std::runtime_error("Specify either the absolute or p-value threshold, not both.");
     if (relSpec && pSpec) throw std::runtime_error("Specify either the relative or p-value threshold, not both.");
 
+    // the device opens while the inputs are read; the scan takes it when the
+    // first group is ready, so input errors (dict, motifs, histograms) are
+    // still reported first
+    DeviceOpener opener;
+    opener.start(device);
+
     std::cout << "Welcome to blamm -- PWM scan module" << std::endl;
     const Settings st = Settings::load();
     st.print();
@@ -444,17 +484,12 @@ int scan_main(int argc, char** argv)
     else if (relSpec) std::cout << "Relative motif score threshold set to: " << relTh << std::endl;
     else std::cout << "P-value motif score threshold set to: " << pvalue << std::endl;
 
-    // The device is opened when the first group is ready to scan, so input
-    // errors (dict, motifs, histograms) are reported first.
     HipContext hc;
     auto open_device = [&]() {
         if (hc.ctx) return;
-        int ndev = 0;
-        blamm_hip_device_count(&ndev);
-        if (ndev == 0) throw std::runtime_error("HIP error: no devices found. Aborting...");
+        hc.ctx = opener.take("device setup");
         std::cout << "Using 1 GPU device (" << (precision == BLAMM_PREC_FP32 ? "fp32" : precision == BLAMM_PREC_BF16X2_LO ? "split-bf16, lazy lo" : "split-bf16")
                   << " MFMA) and " << numThreads << " host thread(s)" << std::endl;
-        hip_check(blamm_hip_create(device, &hc.ctx), "device setup");
     };
 
     const size_t ncols = motifs.size();
