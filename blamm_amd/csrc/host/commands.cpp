@@ -242,7 +242,7 @@ int dict_main(int argc, char** argv)
 // scores of every valid window among the group's first maxLength bases,
 // binned on the device (blamm_hip_hist), one histogram per motif written as
 // writeGNUPlotFile does (hist.cpp:255-263).
-static void empirical_histograms(HipContext& hc, int device, const Group& g, const std::vector<Motif>& motifs,
+static void empirical_histograms(HipContext& hc, DeviceOpener& opener, const Group& g, const std::vector<Motif>& motifs,
                           float pseudocount, size_t numBins, size_t maxLength, size_t lmax,
                           const std::string& histdir)
 {
@@ -259,12 +259,7 @@ static void empirical_histograms(HipContext& hc, int device, const Group& g, con
     }
     if (numBins > (1u << 24)) throw std::runtime_error("number of bins above 2^24");
     const PackedStream s = pack_fasta(g.files, maxLength);
-    if (!hc.ctx) {
-        int ndev = 0;
-        blamm_hip_device_count(&ndev);
-        if (ndev == 0) throw std::runtime_error("HIP error: no devices found. Aborting...");
-        hip_check(blamm_hip_create(device, &hc.ctx), "device setup");
-    }
+    if (!hc.ctx) hc.ctx = opener.take("device setup");
     hip_check(blamm_hip_set_library(hc.ctx, P.data(), (int)ldp, (int)ncols, lengths.data(), thr.data(),
                                     BLAMM_PREC_BF16X2),
               "library upload");
@@ -333,6 +328,10 @@ int hist_main(int argc, char** argv)
             return EXIT_FAILURE;
         }
     }
+    // -e: the device opens while the inputs and the first group's bases are
+    // read; the histograms take it when they need it
+    DeviceOpener opener;
+    if (empirical) opener.start(device);
     std::cout << "Welcome to blamm -- histogram module" << std::endl;
     const Settings st = Settings::load();
     const std::vector<Group> groups = read_dict(std::string(argv[argc - 1]) + ".dict");
@@ -362,7 +361,7 @@ int hist_main(int argc, char** argv)
             std::cout.precision(prec);
         }
         if (empirical) {
-            empirical_histograms(hc, device, g, motifs, st.pseudocount, numBins, maxLength, lmax, histdir);
+            empirical_histograms(hc, opener, g, motifs, st.pseudocount, numBins, maxLength, lmax, histdir);
             continue;
         }
         // one motif per task on numThreads workers (each writes its own
